@@ -1,0 +1,170 @@
+"""Benchmark of the LBM time-step hot path on one MI355X.
+
+Workload: D3Q19 regularized Bingham flow on a 256^3 lattice in FP32, velocity
+Zou-He inlet u_z=0.01 at z=0, pressure Zou-He outlet rho=1 at z=255,
+bounce-back walls on the x and y faces, tau=0.8, yield stress 1e-5, initial
+rho=1, u=(0,0,0.01) + U(-1e-4, 1e-4) per component (seeded). One step = one
+fused gather/BC/collide/store pass over the lattice.
+
+Prints one JSON line: MLUPS over the timed steps (wall clock, inputs resident
+in HBM), the roofline of the step kernel (algorithmic 2*Q*4 = 152 B/node over
+the HIP-event time of the same steps), and a CPU baseline (the oracle's
+OpenMP port of the reference per-node loop on the host cores, bounded sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "viscoplastic-lbm_amd"))
+
+METRIC = "MLUPS + %HBM roofline, D3Q19 256^3 Bingham; sweep jobs/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+N_SIDE = 256
+Q = 19
+STORAGE_BYTES = 4
+
+
+def workload_config(precision="single"):
+    from vlbm.config import SimConfig
+    return SimConfig(precision=precision, NX=N_SIDE, NY=N_SIDE, NZ=N_SIDE, tau=0.8, vel_set="D3Q19",
+                     use_ibm=False, non_newtonian_model="BINGHAM", sim_id=0, path_save="bench",
+                     steps=5000, macr_save=0, data_report=0, builder="squareDuctZouHe",
+                     yield_stress=1e-5, uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0])
+
+
+def initial_fields(seed=12345):
+    shape = (N_SIDE, N_SIDE, N_SIDE)
+    rng = np.random.default_rng(seed)
+    rho = np.ones(shape)
+    ux = rng.uniform(-1e-4, 1e-4, size=shape)
+    uy = rng.uniform(-1e-4, 1e-4, size=shape)
+    uz = 0.01 + rng.uniform(-1e-4, 1e-4, size=shape)
+    return rho, ux, uy, uz
+
+
+def cpu_baseline(init, target_s=15.0):
+    """The oracle (OpenMP restatement of the reference per-node path, FP32
+    storage as the reference's SINGLE_PRECISION build) on the host cores,
+    on a bounded number of steps of the same 256^3 workload."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+    cfg = workload_config("single")
+    o = po.Oracle(po.desc_from(cfg.to_desc()), 4)
+    o.init(*init)
+    threads = po.max_threads()
+    t1 = o.time_steps(1, threads)           # first touch + one step
+    n = max(2, min(200, int(target_s / max(t1, 1e-3))))
+    t = o.time_steps(n, threads)
+    o.close()
+    mlups = N_SIDE ** 3 * n / t / 1e6
+    return {"value": round(mlups, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{n} steps of the 256^3 D3Q19 FP32 Bingham workload in {t:.1f} s "
+                      f"(oracle OpenMP loop, collapse(2) over z,y)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the lattice step has no "
+              "multi-GPU path in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import vlbm
+    try:
+        import torch
+        sync_torch = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only
+        sync_torch = lambda: None
+
+    cfg = workload_config("single")
+    init = initial_fields()
+    lat = vlbm.Lattice(cfg.to_desc())
+    lat.init(*init)
+    lat.step(args.warmup)
+    lat.sync()
+    sync_torch()
+
+    t0 = time.perf_counter()
+    event_ms = lat.time_steps(args.steps)      # HIP events on the library stream around the K steps
+    lat.sync()
+    sync_torch()
+    wall = time.perf_counter() - t0
+
+    nodes = N_SIDE ** 3
+    mlups = nodes * args.steps / wall / 1e6
+    ms_per_step = wall * 1e3 / args.steps
+    kernel_ms = event_ms / args.steps
+    bytes_per_launch = 2 * Q * STORAGE_BYTES * nodes
+    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "step_kernel_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    # sanity of the simulated state after the timed steps
+    lat.step(1, save_last=True)
+    res, mean_rho = lat.reduce()
+    lat.close()
+    if not np.isfinite(mean_rho):
+        print("non-finite state after the timed steps", file=sys.stderr)
+        sys.exit(1)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(init)
+
+    line = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform velocity noise, no dataset)",
+        "config": {
+            "workload": "D3Q19 regularized Bingham, 256x256x256, FP32, Zou-He inlet/outlet z, bounce-back x/y walls",
+            "lattice": [N_SIDE, N_SIDE, N_SIDE],
+            "tau": 0.8, "yield_stress": 1e-5, "u_inlet": 0.01,
+            "storage": "fp32 deviations f_i - w_i, two lattices, pull streaming",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel_ms": round(kernel_ms, 5),
+            "algorithmic_bytes_per_node": 2 * Q * STORAGE_BYTES,
+        },
+        "cpu_baseline": cpu,
+        "mean_rho": mean_rho,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,168 @@
+/*
+ * vlbm.h — C-ABI of the MI355X-native VISCOPLASTIC-LBM time-step library.
+ *
+ * One handle = one lattice resident in HBM. Plain C types only: pointers,
+ * sizes and a POD descriptor. Every call returns VLBM_OK (0) or an error code;
+ * vlbm_last_error() gives the message for the calling thread. This replaces
+ * the reference's checkCudaErrors -> exit(-1) convention
+ * (src/CUDA/errorDef.h:12-47).
+ *
+ * Each entry point names the reference symbol it replaces.
+ */
+#ifndef VLBM_H
+#define VLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define VLBM_OK 0
+#define VLBM_ERR_INVALID 1      /* bad argument or descriptor */
+#define VLBM_ERR_DEVICE 2       /* HIP runtime error */
+#define VLBM_ERR_ALLOC 3        /* device or host allocation failed */
+#define VLBM_ERR_IO 4           /* file could not be written */
+#define VLBM_ERR_UNSUPPORTED 5  /* combination the reference does not define */
+
+/* storage precision of populations (var.h SINGLE_PRECISION / DOUBLE_PRECISION) */
+#define VLBM_FP64 8
+#define VLBM_FP32 4
+#define VLBM_FP16 2             /* FP16 deviations f_i - w_i, FP32 arithmetic */
+
+/* rheology (src/CUDA/NNF/nnf.h:10-30) */
+#define VLBM_NEWTONIAN 0
+#define VLBM_BINGHAM 1
+#define VLBM_POWERLAW 2
+
+/* boundary-condition builders (src/CUDA/boundaryConditionsBuilders/) */
+#define VLBM_BUILDER_TAYLOR_GREEN 0          /* taylorGreenVortex.cu: fully periodic */
+#define VLBM_BUILDER_PLATES_BB_Y 1           /* parallelPlatesBounceBack_Y.cu */
+#define VLBM_BUILDER_PLATES_BB_X 2           /* parallelPlatesBounceBack_X.cu */
+#define VLBM_BUILDER_PLATES_BB_Z 3           /* parallelPlatesBounceBack_Z.cu */
+#define VLBM_BUILDER_SQUARE_DUCT_BB 4        /* squareDuctBounceBack.cu */
+#define VLBM_BUILDER_CLOSED_BOX_BB 5         /* closedBoxBounceBack.cu */
+#define VLBM_BUILDER_PLATES_ZOUHE 6          /* parallelPlatesZouHe.cu */
+#define VLBM_BUILDER_LID_DRIVEN_BB 7         /* lidDrivenBounceBack.cu */
+#define VLBM_BUILDER_COUETTE_ZOUHE 8         /* couetteZouHe.cu */
+#define VLBM_BUILDER_INFLOW_FREE_SLIP 9      /* inflowFreeSlip.cu */
+#define VLBM_BUILDER_PLATES_BB_FREE_SLIP 10  /* parallelPlatesBounceBackFreeSlip.cu */
+#define VLBM_BUILDER_SQUARE_DUCT_ZOUHE 11    /* no reference file: squareDuctBounceBack walls,
+                                                velocity Zou-He inlet at z=0 (uz index 1),
+                                                pressure Zou-He outlet at z=NZ-1 (rho index 0) */
+#define VLBM_BUILDER_CIRCULAR_DUCT_IBB 12    /* circularDuctInterpBounceBack.cu */
+#define VLBM_BUILDER_ANNULAR_DUCT_IBB 13     /* annularDuctInterpBounceBack.cu */
+
+/*
+ * Simulation descriptor: the runtime form of the compile-time constants of
+ * src/CUDA/var.h:18-160, src/CUDA/NNF/nnf.h:10-24 and the builder choice.
+ */
+typedef struct vlbm_desc {
+    int nx, ny, nz;          /* NX, NY, NZ (var.h:97-101); one GPU holds the whole lattice */
+    int q;                   /* 19 or 27 (var.h D3Q19 / D3Q27) */
+    int precision;           /* VLBM_FP64, VLBM_FP32 or VLBM_FP16 */
+    int rheology;            /* VLBM_NEWTONIAN / VLBM_BINGHAM / VLBM_POWERLAW */
+    int builder;             /* VLBM_BUILDER_* */
+    double tau;              /* TAU: Newtonian relaxation time (OMEGA = 1/TAU) */
+    double rho0;             /* RHO_0: initial density */
+    double fx, fy, fz;       /* FX, FY, FZ: constant body force */
+    double eta_p, s_y;       /* Bingham: plastic viscosity ETA_P, yield stress S_Y */
+    double n_index, k_consistency, gamma_0;   /* power law: N_INDEX, K_CONSISTENCY, GAMMA_0 */
+    int omega_clamp;         /* 1: clamp the apparent tau into [tau_min, tau_max] (not in the reference) */
+    double tau_min, tau_max;
+    double ux_bc[8], uy_bc[8], uz_bc[8], rho_bc[8];   /* UX_BC, UY_BC, UZ_BC, RHO_BC (var.h:114-117) */
+    double duct_outer_radius, duct_inner_radius;      /* OUTER_RADIUS, INNER_RADIUS (var.h:141-160) */
+    double outer_rotation, outer_velocity, inner_rotation, inner_velocity;
+    int zouhe_textbook;      /* 0: reproduce the reference's Zou-He lines; 1: textbook Zou-He */
+} vlbm_desc;
+
+typedef struct vlbm_ctx* vlbm_h;
+
+/* Fills a descriptor with the reference defaults of var.h / nnf.h. */
+void vlbm_desc_defaults(vlbm_desc* d);
+
+/* Allocates the lattice on the current device and builds its boundary map.
+ * Replaces Populations::popAllocation + Macroscopics::macrAllocation
+ * (src/CUDA/structs/populations.h:47-52, src/CUDA/structs/macroscopics.h:83-124)
+ * and the gpuBuildBoundaryConditions launch (src/CUDA/main.cu:177-185). */
+int vlbm_create(const vlbm_desc* d, vlbm_h* out);
+void vlbm_destroy(vlbm_h h);
+const char* vlbm_last_error(void);
+
+/* Node-type map words, x fastest ([NZ][NY][NX]), bit-identical to the
+ * reference's NodeTypeMap (src/CUDA/structs/nodeTypeMap.h:15-322) as written
+ * by gpuBuildBoundaryConditions (boundaryConditionsBuilder.h:18-19). Host only. */
+int vlbm_build_map(const vlbm_desc* d, uint32_t* out);
+
+/* Linear indices of the used BC nodes in z,y,x order
+ * (BoundaryConditionsInfo::setupBoundaryConditionsInfo,
+ * src/CUDA/structs/boundaryConditionsInfo.h:76-116). out may be NULL to query the count. */
+int vlbm_bc_nodes(vlbm_h h, uint64_t* out, size_t cap, size_t* count);
+
+/* Equilibrium initialisation from (rho, u) fields of NX*NY*NZ doubles; NULL
+ * pointers mean rho0 / zero velocity (gpuInitialization + gpuMacrInitValue,
+ * src/CUDA/lbmInitialization.cu:48-150). Resets the step counter. */
+int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, const double* uz);
+
+/* Loads / reads populations in the reference layout: Q planes of (NZ+1)*NY*NX
+ * doubles, x fastest (idxPop, src/CUDA/globalFunctions.h:143-147), holding the
+ * post-streaming, post-BC state that the next step collides
+ * (the pop array of src/CUDA/simCheckpoint.cu:216-318). The ghost plane z=NZ
+ * is ignored on load and written as zeros. */
+int vlbm_load_pops(vlbm_h h, const double* pop);
+int vlbm_get_pops(vlbm_h h, double* pop);
+
+/* Power-law apparent omega state, NX*NY*NZ doubles (macr.omega). */
+int vlbm_set_omega(vlbm_h h, const double* omega);
+
+/* Runs n_steps time steps: fused gather + BC + moments + collision.
+ * Replaces gpuMacrCollisionStream + gpuPopulationsTransfer + gpuApplyBC +
+ * swapPop (src/CUDA/main.cu:300-367; src/CUDA/lbm.cu:3-504,579-657).
+ * save_last != 0 stores the macroscopics computed during the last step
+ * (the state entering it), as save_macr_to_array does (src/CUDA/main.cu:296). */
+int vlbm_step(vlbm_h h, int n_steps, int save_last);
+
+/* Computes the macroscopics of the current state into the saved arrays
+ * without advancing (the moments the next step would compute). */
+int vlbm_compute_macroscopics(vlbm_h h);
+
+/* Copies the saved macroscopics to host doubles, [NZ][NY][NX]; any pointer
+ * may be NULL. omega is the apparent omega (non-Newtonian only). */
+int vlbm_macroscopics(vlbm_h h, double* rho, double* ux, double* uy, double* uz, double* omega);
+
+/* Residual and mean density of the saved macroscopics with FP64
+ * accumulation: residual = sum|u - u_old| / sum|u| (1 when the denominator is
+ * 0), mean_rho = sum(rho)/N; u_old becomes u (treatData,
+ * src/CUDA/treatData.cpp:25-78). Either output may be NULL. */
+int vlbm_reduce(vlbm_h h, double* residual, double* mean_rho);
+
+/* Writes PATH/ID/ID_{rho,ux,uy,uz[,omega]}<step>.bin in the storage
+ * precision, [NZ][NY][NX], no header (saveAllMacrBin / getVarFilename,
+ * src/CUDA/lbmReport.cpp:55-172). The folder must exist. */
+int vlbm_save_macr_bin(vlbm_h h, const char* path, const char* id, unsigned step);
+
+/* Writes PATH/ID/ID_info.txt with the keys read by
+ * src/Post Processing/fileTreat.py:56-143 (getSimInfoString,
+ * src/CUDA/lbmReport.cpp:174-386). */
+int vlbm_save_info(vlbm_h h, const char* path, const char* id, unsigned total_steps,
+                   double mlups, double seconds);
+
+/* Steps executed since vlbm_init / vlbm_load_pops. */
+int vlbm_steps_done(vlbm_h h, long long* steps);
+
+/* Runs n_steps and reports the device time between two HIP events recorded
+ * on the library's stream around them (cudaEventRecord timing of main.cu:259-267). */
+int vlbm_time_steps(vlbm_h h, int n_steps, double* ms);
+
+/* Waits for all work queued on the handle's stream. */
+int vlbm_sync(vlbm_h h);
+
+/* Device bytes held by the handle. */
+int vlbm_device_bytes(vlbm_h h, size_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VLBM_H */

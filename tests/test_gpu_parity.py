@@ -1,0 +1,227 @@
+"""GPU parity: libvlbm.so (HIP, gfx950) against the CPU oracle.
+
+Bars: FP64 GPU vs FP64 oracle, rel-L2 of (f - w) <= 1e-10 and of u <= 1e-10;
+FP32 GPU vs FP64 oracle, rel-L2 of rho and u <= 1e-4; FP16 deviations vs FP64
+oracle <= 2e-2 on u (stated per test). Masks / BC lists bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import vlbm
+from helpers import base_cfg, rel_l2, seeded_velocity, weights
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(name):
+    print(f"\n[gpu-case] {name}", flush=True)
+
+
+def run_both(cfg, steps, init=None, prec_gpu=None, save=True):
+    """Same initial state on GPU and oracle; returns (gpu, oracle) after steps."""
+    d = cfg.to_desc()
+    if prec_gpu is not None:
+        d.precision = prec_gpu
+    lat = vlbm.Lattice(d)
+    orc = po.Oracle(po.desc_from(cfg.to_desc()), 8)
+    if init is not None:
+        lat.init(*init)
+        orc.init(*init)
+    else:
+        lat.init()
+        orc.init()
+    if steps:
+        lat.step(steps, save_last=save)
+        orc.step(steps, save_last=save)
+    lat.sync()
+    return lat, orc
+
+
+def pops_dev(p, q):
+    return p[:, :-1] - weights(q)[:, None, None, None]
+
+
+def assert_fp64_parity(lat, orc, q, tol=1e-10, omega=False):
+    gp, op = lat.get_pops(), orc.get_pops()
+    assert rel_l2(pops_dev(gp, q), pops_dev(op, q)) <= tol
+    gm, om = lat.macroscopics(omega=omega), orc.macroscopics()
+    for k in ("ux", "uy", "uz"):
+        if np.linalg.norm(om[k]) > 0:
+            assert rel_l2(gm[k], om[k]) <= tol, k
+        else:
+            assert np.max(np.abs(gm[k])) <= 1e-14
+    assert rel_l2(gm["rho"], om["rho"]) <= tol
+    if omega:
+        assert rel_l2(gm["omega"], om["omega"]) <= tol
+
+
+# ---------------------------------------------------------------------------
+CASES_FP64 = [
+    # name, cfg kwargs, steps, noise
+    ("plates_bingham_c1", dict(NX=4, NY=32, NZ=6, tau=0.9, FX=1e-5, builder="parallelPlatesBounceBack_Y",
+                               non_newtonian_model="BINGHAM", yield_stress=4e-5), 300, 0.0),
+    ("closed_box_q19", dict(NX=8, NY=7, NZ=6, tau=0.7, builder="closedBoxBounceBack"), 60, 1e-2),
+    ("closed_box_q27", dict(NX=8, NY=7, NZ=6, tau=0.7, vel_set="D3Q27", builder="closedBoxBounceBack"), 60, 1e-2),
+    ("square_duct_zouhe_c2", dict(NX=12, NY=10, NZ=16, tau=0.8, builder="squareDuctZouHe",
+                                  non_newtonian_model="BINGHAM", yield_stress=1e-5,
+                                  uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0]), 150, 1e-4),
+    ("plates_zouhe", dict(NX=6, NY=8, NZ=10, tau=0.8, builder="parallelPlatesZouHe",
+                          rho_bc=[1, 1.001, 0.999, 1, 1, 1, 1, 1]), 120, 1e-3),
+    ("lid_driven", dict(NX=9, NY=9, NZ=5, tau=0.7, builder="lidDrivenBounceBack", u_max=0.05), 120, 0.0),
+    ("couette_zouhe", dict(NX=5, NY=9, NZ=4, tau=0.8, builder="couetteZouHe", u_max=0.02), 120, 0.0),
+    ("inflow_free_slip", dict(NX=5, NY=8, NZ=9, tau=0.8, builder="inflowFreeSlip", u_max=0.02), 120, 1e-3),
+    ("plates_bb_free_slip_q27", dict(NX=6, NY=8, NZ=5, tau=0.8, FZ=1e-5, vel_set="D3Q27",
+                                     builder="parallelPlatesBounceBackFreeSlip"), 150, 1e-3),
+    ("plates_x_q27", dict(NX=9, NY=5, NZ=4, tau=0.6, FY=2e-5, vel_set="D3Q27",
+                          builder="parallelPlatesBounceBack_X"), 100, 1e-3),
+    ("plates_z", dict(NX=5, NY=4, NZ=9, tau=0.6, FX=2e-5, builder="parallelPlatesBounceBack_Z"), 100, 1e-3),
+    ("square_duct_bb", dict(NX=7, NY=8, NZ=5, tau=0.9, FZ=1e-5, builder="squareDuctBounceBack"), 100, 1e-3),
+    ("taylor_green_q27_power_law", dict(NX=8, NY=8, NZ=8, tau=0.7, vel_set="D3Q27", builder="taylorGreenVortex",
+                                        non_newtonian_model="POWERLAW", n_index=0.6, k_consistency=0.05,
+                                        tau_clamp=[0.51, 2.0]), 80, 5e-3),
+    ("plates_power_law_q27", dict(NX=4, NY=16, NZ=4, tau=0.7, FX=5e-6, vel_set="D3Q27",
+                                  builder="parallelPlatesBounceBack_Y", non_newtonian_model="POWERLAW",
+                                  n_index=0.6, k_consistency=0.02, tau_clamp=[0.51, 2.0]), 200, 1e-4),
+    ("circular_duct_ibb", dict(NX=18, NY=18, NZ=4, tau=0.8, FZ=1e-5, builder="circularDuctInterpBounceBack"),
+     120, 0.0),
+    ("annular_duct_ibb_q27", dict(NX=22, NY=22, NZ=3, tau=0.8, FZ=1e-5, vel_set="D3Q27",
+                                  builder="annularDuctInterpBounceBack", inner_rotation=0.002), 80, 0.0),
+]
+
+
+@pytest.mark.parametrize("name,kw,steps,noise", CASES_FP64, ids=[c[0] for c in CASES_FP64])
+def test_fp64_matches_oracle(name, kw, steps, noise):
+    announce(name)
+    cfg = base_cfg(**kw)
+    shape = (cfg.NZ, cfg.NY, cfg.NX)
+    init = None
+    if noise:
+        u0 = (0.0, 0.0, 0.01) if "zouhe_c2" in name else (0.0, 0.0, 0.0)
+        init = (np.ones(shape),) + seeded_velocity(shape, u0, noise, seed=12345)
+    lat, orc = run_both(cfg, steps, init)
+    q = 19 if cfg.vel_set == "D3Q19" else 27
+    assert_fp64_parity(lat, orc, q, omega=cfg.non_newtonian_model is not None)
+    # the BC node list matches the reference's scan
+    assert np.array_equal(lat.bc_nodes(), orc.bc_list())
+    lat.close()
+
+
+@pytest.mark.parametrize("name,kw,steps,noise", [c for c in CASES_FP64 if c[0] in
+                                                  ("plates_bingham_c1", "square_duct_zouhe_c2", "closed_box_q27",
+                                                   "plates_power_law_q27", "circular_duct_ibb")],
+                         ids=lambda v: v if isinstance(v, str) else "")
+def test_fp32_within_1e4_of_fp64_oracle(name, kw, steps, noise):
+    announce("fp32 " + name)
+    cfg = base_cfg(**kw)
+    shape = (cfg.NZ, cfg.NY, cfg.NX)
+    init = None
+    if noise:
+        u0 = (0.0, 0.0, 0.01) if "zouhe_c2" in name else (0.0, 0.0, 0.0)
+        init = (np.ones(shape),) + seeded_velocity(shape, u0, noise, seed=7)
+    lat, orc = run_both(cfg, steps, init, prec_gpu=vlbm.FP32)
+    gm, om = lat.macroscopics(), orc.macroscopics()
+    assert rel_l2(gm["rho"], om["rho"]) <= 1e-4
+    u_g = np.stack([gm["ux"], gm["uy"], gm["uz"]])
+    u_o = np.stack([om["ux"], om["uy"], om["uz"]])
+    assert rel_l2(u_g, u_o) <= 1e-4
+    lat.close()
+
+
+def test_fp16_deviation_storage_plates():
+    """FP16 storage of f - w with FP32 arithmetic: Poiseuille plates within 2e-2 rel-L2 of FP64."""
+    announce("fp16 plates")
+    cfg = base_cfg(NX=4, NY=24, NZ=4, tau=0.8, FX=1e-5, builder="parallelPlatesBounceBack_Y")
+    lat, orc = run_both(cfg, 1500, None, prec_gpu=vlbm.FP16)
+    gm, om = lat.macroscopics(), orc.macroscopics()
+    assert rel_l2(gm["ux"], om["ux"]) <= 2e-2
+    lat.close()
+
+
+def test_state_roundtrip_and_on_demand_moments():
+    """load_pops/get_pops round trip; moments on demand equal the next step's saved ones."""
+    announce("roundtrip")
+    cfg = base_cfg(NX=6, NY=7, NZ=5, tau=0.8, FX=1e-5, builder="squareDuctBounceBack")
+    shape = (5, 7, 6)
+    lat, orc = run_both(cfg, 25, (np.ones(shape),) + seeded_velocity(shape, (0, 0, 0), 1e-2, 3))
+    pops = orc.get_pops()
+    lat.load_pops(pops)
+    back = lat.get_pops()
+    assert np.max(np.abs(back[:, :-1] - pops[:, :-1])) < 1e-15
+    lat.step(3)
+    lat.compute_macroscopics()
+    m1 = lat.macroscopics()
+    lat.step(1, save_last=True)
+    m2 = lat.macroscopics()
+    for k in m1:
+        assert np.array_equal(m1[k], m2[k])
+    lat.close()
+
+
+def test_reduce_matches_numpy():
+    announce("reduce")
+    cfg = base_cfg(NX=10, NY=9, NZ=8, tau=0.8, FX=1e-5, builder="parallelPlatesBounceBack_Y")
+    lat = vlbm.Lattice(cfg.to_desc())
+    lat.init()
+    lat.step(40, save_last=True)
+    m0 = lat.macroscopics()
+    res0, rho0 = lat.reduce()
+    lat.step(40, save_last=True)
+    m1 = lat.macroscopics()
+    res1, rho1 = lat.reduce()
+    u0 = np.stack([m0["ux"], m0["uy"], m0["uz"]])
+    u1 = np.stack([m1["ux"], m1["uy"], m1["uz"]])
+    exp1 = np.sum(np.sqrt(np.sum((u1 - u0) ** 2, axis=0))) / np.sum(np.sqrt(np.sum(u1 ** 2, axis=0)))
+    assert np.isclose(res1, exp1, rtol=1e-12)
+    assert np.isclose(rho1, m1["rho"].mean(), rtol=1e-14)
+    assert res0 == 1.0 or res0 > 0
+    lat.close()
+
+
+def test_output_files_layout(tmp_path):
+    """ID_rho<step>.bin raw [NZ][NY][NX] float, ID_info.txt keys of fileTreat.py."""
+    announce("output files")
+    cfg = base_cfg(precision="single", NX=5, NY=6, NZ=7, tau=0.8, FX=1e-5, builder="parallelPlatesBounceBack_Y")
+    lat = vlbm.Lattice(cfg.to_desc())
+    lat.init()
+    lat.step(10, save_last=True)
+    (tmp_path / "001").mkdir()
+    lat.save_macr_bin(str(tmp_path), "001", 123)
+    lat.save_info(str(tmp_path), "001", 10, 1.0, 1.0)
+    m = lat.macroscopics()
+    raw = np.fromfile(tmp_path / "001" / "001_ux000123.bin", dtype="f")
+    assert raw.size == 5 * 6 * 7
+    assert np.array_equal(raw.reshape(7, 6, 5), m["ux"].astype(np.float32))
+    info = (tmp_path / "001" / "001_info.txt").read_text()
+    for key in ("Precision: float", "NX: 5", "NY: 6", "NZ: 7", "NZ_TOTAL: 7", "Nsteps: 10"):
+        assert key in info
+    lat.close()
+
+
+# ---------------------------------------------------------------------------
+# full-size workload: D3Q19 256^3 regularized Bingham, Zou-He inlet/outlet
+# ---------------------------------------------------------------------------
+def c2_config(precision):
+    return base_cfg(precision=precision, NX=256, NY=256, NZ=256, tau=0.8, builder="squareDuctZouHe",
+                    non_newtonian_model="BINGHAM", yield_stress=1e-5, uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0])
+
+
+def test_c2_fp32_vs_fp64_full_size():
+    """At 256^3 the FP32 path stays within 1e-4 rel-L2 (rho, u) of the FP64 GPU
+    path, which matches the oracle at small sizes (test_fp64_matches_oracle)."""
+    announce("c2 256^3 fp32 vs fp64, 300 steps")
+    shape = (256, 256, 256)
+    init = (np.ones(shape),) + seeded_velocity(shape, (0, 0, 0.01), 1e-4, seed=12345)
+    out = {}
+    for prec in ("single", "double"):
+        lat = vlbm.Lattice(c2_config(prec).to_desc())
+        lat.init(*init)
+        lat.step(300, save_last=True)
+        out[prec] = lat.macroscopics()
+        lat.close()
+    assert rel_l2(out["single"]["rho"], out["double"]["rho"]) <= 1e-4
+    u32 = np.stack([out["single"][k] for k in ("ux", "uy", "uz")])
+    u64 = np.stack([out["double"][k] for k in ("ux", "uy", "uz")])
+    assert rel_l2(u32, u64) <= 1e-4

@@ -1,0 +1,793 @@
+// C-ABI of the MI355X LBM library: lattice handles, boundary-map builders,
+// initialisation, stepping, reductions and the reference's output layout.
+// Declarations and the reference symbol each entry point replaces are in
+// include/vlbm.h.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/vlbm.h"
+#include "lattice.h"
+
+using namespace vlbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(VLBM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Boundary-map builders.
+//
+// Box builders are described by what sits on each wall face; the node word of
+// a region (corner / edge / face / interior, classified in the reference's
+// if-chain priority) follows from the faces of that region. Duct builders
+// evaluate the circle geometry per node.
+// ---------------------------------------------------------------------------
+constexpr unsigned kRegionFaces[27] = {
+    fS | fW | fB, fS | fW | fF, fS | fE | fB, fS | fE | fF, fN | fW | fB, fN | fW | fF, fN | fE | fB, fN | fE | fF,
+    fS | fW, fS | fE, fN | fW, fN | fE,
+    fS | fB, fS | fF, fN | fB, fN | fF, fW | fB, fW | fF, fE | fB, fE | fF,
+    fS, fN, fW, fE, fB, fF, 0};
+
+int host_region(int x, int y, int z, int nx, int ny, int nz) {
+    const bool s = (y == 0), n = (y == ny - 1), w = (x == 0), e = (x == nx - 1);
+    const bool b = (z == 0), f = (z == nz - 1);
+    const bool c[27] = {s && w && b, s && w && f, s && e && b, s && e && f, n && w && b, n && w && f, n && e && b,
+                        n && e && f, s && w, s && e, n && w, n && e, s && b, s && f, n && b, n && f, w && b, w && f,
+                        e && b, e && f, s, n, w, e, b, f, true};
+    for (int r = 0; r < 27; r++)
+        if (c[r]) return r;
+    return 26;
+}
+
+int dir_of_faces(unsigned faces) {
+    for (int d = 0; d < 26; d++)
+        if (kDirFaces[d] == faces) return d;
+    return -1;
+}
+
+uint32_t plain_word() { return make_word(true, false, kNull, 0, 0, 0, 0, 0, 0); }
+uint32_t bc_word(int scheme, int dir, int uxi = 0, int uyi = 0, int uzi = 0, int rhoi = 0) {
+    return make_word(true, false, scheme, dir, 0, uxi, uyi, uzi, rhoi);
+}
+
+bool is_box_builder(int b) { return b >= VLBM_BUILDER_TAYLOR_GREEN && b <= VLBM_BUILDER_SQUARE_DUCT_ZOUHE; }
+bool is_duct_builder(int b) { return b == VLBM_BUILDER_CIRCULAR_DUCT_IBB || b == VLBM_BUILDER_ANNULAR_DUCT_IBB; }
+
+// Node word of each box region for a builder.
+bool box_region_words(int builder, uint32_t out[27]) {
+    for (int r = 0; r < 27; r++) {
+        const unsigned rf = kRegionFaces[r];
+        uint32_t w = plain_word();
+        auto walls = [&](unsigned wallset) -> bool {
+            const unsigned hit = rf & wallset;
+            if (!hit) return false;
+            w = bc_word(kBounceBack, dir_of_faces(hit));
+            return true;
+        };
+        switch (builder) {
+        case VLBM_BUILDER_TAYLOR_GREEN: break;
+        case VLBM_BUILDER_PLATES_BB_Y: walls(fN | fS); break;
+        case VLBM_BUILDER_PLATES_BB_X: walls(fW | fE); break;
+        case VLBM_BUILDER_PLATES_BB_Z: walls(fB | fF); break;
+        case VLBM_BUILDER_SQUARE_DUCT_BB: walls(fN | fS | fW | fE); break;
+        case VLBM_BUILDER_CLOSED_BOX_BB: walls(fN | fS | fW | fE | fF | fB); break;
+        case VLBM_BUILDER_SQUARE_DUCT_ZOUHE:
+            if (!walls(fN | fS | fW | fE)) {
+                if (rf & fB) w = bc_word(kVelZouHe, dB, 0, 0, 1, 0);
+                else if (rf & fF) w = bc_word(kPresZouHe, dF, 0, 0, 0, 0);
+            }
+            break;
+        case VLBM_BUILDER_PLATES_ZOUHE:
+            if (rf & fS) w = bc_word(kVelZouHe, dS);
+            else if (rf & fN) w = bc_word(kVelZouHe, dN);
+            else if (rf & fB) w = bc_word(kPresZouHe, dB, 0, 0, 0, 1);
+            else if (rf & fF) w = bc_word(kPresZouHe, dF, 0, 0, 0, 2);
+            break;
+        case VLBM_BUILDER_LID_DRIVEN_BB:
+            // moving lid on N (ux index 1); lid edges and corners stay plain bounce-back
+            if ((rf & (fN | fS | fW | fE)) == fN) w = bc_word(kVelBB, dN, 1, 0, 0, 0);
+            else walls(fN | fS | fW | fE);
+            break;
+        case VLBM_BUILDER_COUETTE_ZOUHE:
+            if (rf & fS) w = bc_word(kVelZouHe, dS, 0, 0, 2, 0);
+            else if (rf & fN) w = bc_word(kVelZouHe, dN, 0, 0, 1, 0);
+            break;
+        case VLBM_BUILDER_INFLOW_FREE_SLIP:
+            if (rf & fS) w = bc_word(kFreeSlip, dS);
+            else if (rf & fN) w = bc_word(kFreeSlip, dN);
+            else if (rf & fB) w = bc_word(kVelBB, dB, 0, 0, 1, 0);
+            else if (rf & fF) w = bc_word(kVelBB, dF, 0, 0, 1, 0);
+            break;
+        case VLBM_BUILDER_PLATES_BB_FREE_SLIP:
+            if (rf & fS) w = bc_word(kBounceBack, dS);
+            else if (rf & fN) w = bc_word(kFreeSlip, dN);
+            break;
+        default: return false;
+        }
+        out[r] = w;
+    }
+    return true;
+}
+
+// distPoints2D of the reference: only the first difference is cast to float
+// (src/CUDA/globalFunctions.h:158-162). R is the build's dfloat.
+template <typename R>
+R ref_dist2d(R x1, R y1, R x2, R y2) {
+    return (R)std::sqrt((float)(x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
+}
+
+// Circular / annular duct words (circularDuctInterpBounceBack.cu:35-121,
+// annularDuctInterpBounceBack.cu): nodes outside the wall are unused, nodes
+// within sqrt(2) of a wall are interpolated-bounce-back nodes with the D2Q9
+// set of populations whose source lies outside, and the next ring keeps its
+// post-collision populations.
+template <typename R>
+void duct_words(const vlbm_desc& d, std::vector<uint32_t>& out) {
+    const bool annular = d.builder == VLBM_BUILDER_ANNULAR_DUCT_IBB;
+    const R Ro = (R)d.duct_outer_radius, Ri = (R)d.duct_inner_radius;
+    const R xc = (R)(((d.nx - 1) / 2.0) + 0.5), yc = (R)(((d.ny - 1) / 2.0) + 0.5);
+    const float s2 = std::sqrt((float)2);
+    const int dxs[8] = {1, 0, -1, 0, 1, -1, -1, 1}, dys[8] = {0, 1, 0, -1, 1, 1, -1, -1};
+    std::vector<uint32_t> plane((size_t)d.nx * d.ny);
+    for (int y = 0; y < d.ny; y++)
+        for (int x = 0; x < d.nx; x++) {
+            const R xn = (R)(x + 0.5), yn = (R)(y + 0.5);
+            const R dist = ref_dist2d<R>(xn, yn, xc, yc);
+            bool used = true, save = false;
+            int scheme = kNull, dir = annular ? dF : dN;
+            bool mark_unknowns = annular;
+            auto outside = [&](R dd) { return annular ? (dd > Ro || dd < Ri) : (dd > Ro); };
+            if (outside(dist)) {
+                used = false;
+            } else if (dist > (Ro - s2)) {
+                scheme = kInterpBB; dir = dN; mark_unknowns = true;
+            } else if (dist > (Ro - 2 * s2)) {
+                save = true;
+            } else if (annular && dist < Ri + s2) {
+                scheme = kInterpBB; dir = dS;
+            } else if (annular && dist < Ri + 2 * s2) {
+                save = true;
+            }
+            uint32_t w = make_word(used, save, scheme, dir, 0, 0, 0, 0, 0);
+            if (mark_unknowns) {
+                for (int i = 0; i < 8; i++) {
+                    const R xa = xn - dxs[i], ya = yn - dys[i];
+                    bool unknown;
+                    if (xa < d.nx && xa > 0 && ya < d.ny && ya > 0) unknown = outside(ref_dist2d<R>(xa, ya, xc, yc));
+                    else unknown = true;
+                    if (unknown) w |= (1u << i) << kOffInterp;
+                }
+            }
+            plane[(size_t)y * d.nx + x] = w;
+        }
+    out.resize((size_t)d.nx * d.ny * d.nz);
+    for (int z = 0; z < d.nz; z++) std::memcpy(&out[(size_t)z * d.nx * d.ny], plane.data(), plane.size() * 4);
+}
+
+int validate(const vlbm_desc* d) {
+    if (!d) return fail(VLBM_ERR_INVALID, "descriptor is NULL");
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(VLBM_ERR_INVALID, "NX, NY and NZ must be >= 1");
+    const double n = (double)d->nx * d->ny * d->nz;
+    if (n >= 2147483648.0) return fail(VLBM_ERR_INVALID, "lattice has 2^31 nodes or more");
+    if (d->q != 19 && d->q != 27) return fail(VLBM_ERR_INVALID, "velocity set must be D3Q19 or D3Q27");
+    if (d->precision != VLBM_FP64 && d->precision != VLBM_FP32 && d->precision != VLBM_FP16)
+        return fail(VLBM_ERR_INVALID, "precision must be 8, 4 or 2 bytes");
+    if (d->rheology < 0 || d->rheology > 2) return fail(VLBM_ERR_INVALID, "unknown rheology");
+    if (!is_box_builder(d->builder) && !is_duct_builder(d->builder))
+        return fail(VLBM_ERR_INVALID, "unknown boundary-condition builder");
+    if (!(d->tau > 0.5) && d->rheology == VLBM_NEWTONIAN) return fail(VLBM_ERR_INVALID, "tau must exceed 0.5");
+    if (d->rheology == VLBM_BINGHAM && !(d->eta_p > 0)) return fail(VLBM_ERR_INVALID, "eta_p must be > 0");
+    if (d->rheology == VLBM_POWERLAW && !(d->k_consistency > 0)) return fail(VLBM_ERR_INVALID, "K must be > 0");
+    if (d->omega_clamp && !(d->tau_min > 0.5 && d->tau_max >= d->tau_min))
+        return fail(VLBM_ERR_INVALID, "tau clamp needs 0.5 < tau_min <= tau_max");
+    if (d->q == 27 && d->builder != VLBM_BUILDER_TAYLOR_GREEN) {
+        // Zou-He and velocity bounce-back exist for D3Q19 only in the reference
+        // (boundaryConditionsHandler.cu:274,318,361); refuse instead of silently skipping.
+        uint32_t words[27];
+        if (is_box_builder(d->builder) && box_region_words(d->builder, words)) {
+            for (int r = 0; r < 27; r++) {
+                const int s = w_scheme(words[r]);
+                if (s == kVelZouHe || s == kPresZouHe || s == kVelBB)
+                    return fail(VLBM_ERR_UNSUPPORTED, "Zou-He and velocity bounce-back are D3Q19-only");
+            }
+        }
+    }
+    if (is_duct_builder(d->builder)) {
+        if (!(d->duct_outer_radius > 0)) return fail(VLBM_ERR_INVALID, "duct radius must be > 0");
+        if (d->builder == VLBM_BUILDER_ANNULAR_DUCT_IBB && !(d->duct_inner_radius > 0 && d->duct_inner_radius < d->duct_outer_radius))
+            return fail(VLBM_ERR_INVALID, "annular duct needs 0 < inner radius < outer radius");
+    }
+    return VLBM_OK;
+}
+
+int build_words(const vlbm_desc& d, std::vector<uint32_t>& out) {
+    const size_t n = (size_t)d.nx * d.ny * d.nz;
+    if (is_duct_builder(d.builder)) {
+        if (d.precision == VLBM_FP64) duct_words<double>(d, out);
+        else duct_words<float>(d, out);
+        return VLBM_OK;
+    }
+    uint32_t words[27];
+    if (!box_region_words(d.builder, words)) return fail(VLBM_ERR_INVALID, "unknown builder");
+    out.resize(n);
+    size_t k = 0;
+    for (int z = 0; z < d.nz; z++)
+        for (int y = 0; y < d.ny; y++)
+            for (int x = 0; x < d.nx; x++) out[k++] = words[host_region(x, y, z, d.nx, d.ny, d.nz)];
+    return VLBM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------
+struct vlbm_ctx {
+    vlbm_desc d;
+    size_t n = 0;             // nodes
+    int64_t stride = 0;       // elements between population planes
+    int sbytes = 0;           // storage bytes per population
+    int tbytes = 0;           // compute / macroscopic bytes per value
+    void* lat[2] = {nullptr, nullptr};
+    int cur = 0;              // lattice holding the current state
+    bool direct = true;       // current state is post-stream (after init / load)
+    long long steps = 0;
+    uint32_t* d_mask = nullptr;
+    std::vector<uint32_t> words;   // host map
+    uint32_t region_word[27];
+    bool mask_geom = false;
+    void* omega_state = nullptr;
+    void* macr[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // rho, ux, uy, uz, omega
+    void* uold[3] = {nullptr, nullptr, nullptr};
+    double* partials = nullptr;
+    double* out3 = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t device_bytes = 0;
+};
+
+namespace {
+
+template <typename T>
+void fill_params(const vlbm_ctx* h, Params<T>& p) {
+    std::memset(&p, 0, sizeof(p));
+    const vlbm_desc& d = h->d;
+    p.nx = d.nx; p.ny = d.ny; p.nz = d.nz;
+    p.n = (uint32_t)h->n;
+    p.div_nx = make_fastdiv((uint32_t)d.nx);
+    p.div_nxy = make_fastdiv((uint32_t)(d.nx * d.ny));
+    p.stride = h->stride;
+    p.mask = h->d_mask;
+    for (int r = 0; r < 27; r++) p.region_word[r] = h->region_word[r];
+    for (int k = 0; k < 8; k++) {
+        p.ux_bc[k] = (T)d.ux_bc[k]; p.uy_bc[k] = (T)d.uy_bc[k];
+        p.uz_bc[k] = (T)d.uz_bc[k]; p.rho_bc[k] = (T)d.rho_bc[k];
+    }
+    p.fx = (T)d.fx; p.fy = (T)d.fy; p.fz = (T)d.fz;
+    p.omega = (T)(1.0 / d.tau);
+    p.omega_p = (T)(1.0 / (3.0 * d.eta_p + 0.5));
+    p.s_y = (T)d.s_y;
+    p.k_cons = (T)d.k_consistency;
+    p.n_minus_1 = (T)(d.n_index - 1.0);
+    p.gamma_0 = (T)d.gamma_0;
+    p.clamp = d.omega_clamp;
+    if (d.omega_clamp) { p.omega_lo = (T)(1.0 / d.tau_max); p.omega_hi = (T)(1.0 / d.tau_min); }
+    p.omega_state = (T*)h->omega_state;
+    p.m_rho = (T*)h->macr[0]; p.m_ux = (T*)h->macr[1]; p.m_uy = (T*)h->macr[2];
+    p.m_uz = (T*)h->macr[3]; p.m_omega = (T*)h->macr[4];
+    p.zouhe_textbook = d.zouhe_textbook;
+    p.duct_R = (T)d.duct_outer_radius; p.duct_r = (T)d.duct_inner_radius;
+    p.w_out = (T)d.outer_rotation; p.u_out = (T)d.outer_velocity;
+    p.w_in = (T)d.inner_rotation; p.u_in = (T)d.inner_velocity;
+}
+
+// Runs one kernel with parameters of the handle's compute type.
+template <typename F>
+int with_params(vlbm_ctx* h, F&& fn) {
+    if (h->tbytes == 8) {
+        Params<double> p;
+        fill_params(h, p);
+        return fn((void*)&p, p);
+    }
+    Params<float> p;
+    fill_params(h, p);
+    return fn((void*)&p, p);
+}
+
+int check_launch(int rc, const char* what) {
+    if (rc != 0) {
+        hipError_t e = hipGetLastError();
+        return fail(VLBM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    }
+    return VLBM_OK;
+}
+
+void release(vlbm_ctx* h) {
+    if (!h) return;
+    for (void* p : h->lat) if (p) (void)hipFree(p);
+    for (void* p : h->macr) if (p) (void)hipFree(p);
+    for (void* p : h->uold) if (p) (void)hipFree(p);
+    if (h->omega_state) (void)hipFree(h->omega_state);
+    if (h->d_mask) (void)hipFree(h->d_mask);
+    if (h->partials) (void)hipFree(h->partials);
+    if (h->out3) (void)hipFree(h->out3);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int dalloc(vlbm_ctx* h, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(VLBM_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    }
+    h->device_bytes += bytes;
+    return VLBM_OK;
+}
+
+// Host copy of a macroscopic field (compute type) as doubles.
+int fetch_field(vlbm_ctx* h, const void* dev, double* out) {
+    if (h->tbytes == 8) {
+        HIP_TRY(hipMemcpyAsync(out, dev, h->n * 8, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return VLBM_OK;
+    }
+    std::vector<float> tmp(h->n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), dev, h->n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (size_t k = 0; k < h->n; k++) out[k] = tmp[k];
+    return VLBM_OK;
+}
+
+int upload_field(vlbm_ctx* h, void* dev, const double* in) {
+    if (h->tbytes == 8) {
+        HIP_TRY(hipMemcpyAsync(dev, in, h->n * 8, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return VLBM_OK;
+    }
+    std::vector<float> tmp(h->n);
+    for (size_t k = 0; k < h->n; k++) tmp[k] = (float)in[k];
+    HIP_TRY(hipMemcpyAsync(dev, tmp.data(), h->n * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return VLBM_OK;
+}
+
+double pop_weight(int q, int i) { return q == 19 ? weight<19>(i) : weight<27>(i); }
+
+}  // namespace
+
+extern "C" {
+
+const char* vlbm_last_error(void) { return g_err.c_str(); }
+
+void vlbm_desc_defaults(vlbm_desc* d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    // src/CUDA/var.h:97-117 and src/CUDA/NNF/nnf.h:10-24
+    d->nx = 128; d->ny = 128; d->nz = 384;
+    d->q = 19;
+    d->precision = VLBM_FP32;
+    d->rheology = VLBM_NEWTONIAN;
+    d->builder = VLBM_BUILDER_CLOSED_BOX_BB;
+    d->tau = 0.6;
+    d->rho0 = 1.0;
+    d->eta_p = 0.3; d->s_y = 1e-4;
+    d->n_index = 0.5; d->k_consistency = 1.0 * (0.6 - 0.5) / 3; d->gamma_0 = 0;
+    d->tau_min = 0.51; d->tau_max = 2.0;
+    const double umax = 0.1;
+    const double ux[8] = {0, umax, 0, 0, 0, 0, 0, 0};
+    const double uy[8] = {0, umax / 2, -umax / 2, 0, 0, 0, 0, 0};
+    const double uz[8] = {0, umax, -umax, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 8; k++) {
+        d->ux_bc[k] = ux[k]; d->uy_bc[k] = uy[k]; d->uz_bc[k] = uz[k]; d->rho_bc[k] = 1.0;
+    }
+    d->duct_outer_radius = d->ny / 2.0 - 1.5;
+    d->duct_inner_radius = d->duct_outer_radius / 4.0;
+}
+
+int vlbm_build_map(const vlbm_desc* d, uint32_t* out) {
+    int rc = validate(d);
+    if (rc) return rc;
+    if (!out) return fail(VLBM_ERR_INVALID, "output is NULL");
+    std::vector<uint32_t> w;
+    rc = build_words(*d, w);
+    if (rc) return rc;
+    std::memcpy(out, w.data(), w.size() * 4);
+    return VLBM_OK;
+}
+
+int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
+    if (!out) return fail(VLBM_ERR_INVALID, "output handle pointer is NULL");
+    *out = nullptr;
+    int rc = validate(d);
+    if (rc) return rc;
+    vlbm_ctx* h = new (std::nothrow) vlbm_ctx;
+    if (!h) return fail(VLBM_ERR_ALLOC, "host allocation failed");
+    h->d = *d;
+    h->n = (size_t)d->nx * d->ny * d->nz;
+    h->stride = (int64_t)h->n;
+    h->sbytes = d->precision;
+    h->tbytes = d->precision == VLBM_FP64 ? 8 : 4;
+    rc = build_words(*d, h->words);
+    if (rc) { release(h); return rc; }
+    h->mask_geom = is_duct_builder(d->builder);
+    if (!h->mask_geom) box_region_words(d->builder, h->region_word);
+    else for (int r = 0; r < 27; r++) h->region_word[r] = 0;
+
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { release(h); return fail(VLBM_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        release(h);
+        return fail(VLBM_ERR_DEVICE, "hipEventCreate failed");
+    }
+    const size_t lat_bytes = (size_t)d->q * h->n * h->sbytes;
+    for (int k = 0; k < 2; k++) {
+        if ((rc = dalloc(h, &h->lat[k], lat_bytes))) { release(h); return rc; }
+        if (hipMemsetAsync(h->lat[k], 0, lat_bytes, h->stream) != hipSuccess) { release(h); return fail(VLBM_ERR_DEVICE, "hipMemset"); }
+    }
+    for (int k = 0; k < 5; k++) {
+        if (k == 4 && d->rheology == VLBM_NEWTONIAN) continue;
+        if ((rc = dalloc(h, &h->macr[k], h->n * h->tbytes))) { release(h); return rc; }
+    }
+    for (int k = 0; k < 3; k++)
+        if ((rc = dalloc(h, &h->uold[k], h->n * h->tbytes))) { release(h); return rc; }
+    if (d->rheology == VLBM_POWERLAW)
+        if ((rc = dalloc(h, &h->omega_state, h->n * h->tbytes))) { release(h); return rc; }
+    if (h->mask_geom) {
+        if ((rc = dalloc(h, (void**)&h->d_mask, h->n * 4))) { release(h); return rc; }
+        e = hipMemcpy(h->d_mask, h->words.data(), h->n * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { release(h); return fail(VLBM_ERR_DEVICE, "mask upload failed"); }
+    }
+    if ((rc = dalloc(h, (void**)&h->partials, 1024 * 3 * sizeof(double)))) { release(h); return rc; }
+    if ((rc = dalloc(h, (void**)&h->out3, 3 * sizeof(double)))) { release(h); return rc; }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) { release(h); return fail(VLBM_ERR_DEVICE, "stream sync failed"); }
+    rc = vlbm_init(h, nullptr, nullptr, nullptr, nullptr);
+    if (rc) { release(h); return rc; }
+    *out = h;
+    return VLBM_OK;
+}
+
+void vlbm_destroy(vlbm_h h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    release(h);
+}
+
+int vlbm_bc_nodes(vlbm_h h, uint64_t* out, size_t cap, size_t* count) {
+    if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
+    size_t k = 0;
+    for (size_t i = 0; i < h->n; i++) {
+        const uint32_t w = h->words[i];
+        if (w_used(w) && w_scheme(w) != kNull) {
+            if (out && k < cap) out[k] = i;
+            k++;
+        }
+    }
+    if (count) *count = k;
+    if (out && k > cap) return fail(VLBM_ERR_INVALID, "output capacity too small");
+    return VLBM_OK;
+}
+
+int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, const double* uz) {
+    if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
+    const size_t n = h->n;
+    // macroscopic fields as doubles on the device for the equilibrium kernel
+    std::vector<double> r(n), vx(n), vy(n), vz(n);
+    for (size_t k = 0; k < n; k++) {
+        r[k] = rho ? rho[k] : h->d.rho0;
+        vx[k] = ux ? ux[k] : 0.0;
+        vy[k] = uy ? uy[k] : 0.0;
+        vz[k] = uz ? uz[k] : 0.0;
+    }
+    double* dev = nullptr;
+    HIP_TRY(hipMalloc((void**)&dev, 4 * n * sizeof(double)));
+    int rc = VLBM_OK;
+    do {
+        if (hipMemcpyAsync(dev, r.data(), n * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemcpyAsync(dev + n, vx.data(), n * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemcpyAsync(dev + 2 * n, vy.data(), n * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemcpyAsync(dev + 3 * n, vz.data(), n * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+            rc = fail(VLBM_ERR_DEVICE, "upload of initial fields failed");
+            break;
+        }
+        h->cur = 0;
+        if (launch_fill_feq(h->d.q, h->sbytes, h->lat[0], h->stride, (uint32_t)n, dev, dev + n, dev + 2 * n,
+                            dev + 3 * n, h->stream)) {
+            rc = check_launch(-1, "equilibrium kernel");
+            break;
+        }
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = fail(VLBM_ERR_DEVICE, "init sync failed"); break; }
+    } while (0);
+    (void)hipFree(dev);
+    if (rc) return rc;
+    // saved macroscopics and the residual reference start at the initial fields
+    if ((rc = upload_field(h, h->macr[0], r.data()))) return rc;
+    if ((rc = upload_field(h, h->macr[1], vx.data()))) return rc;
+    if ((rc = upload_field(h, h->macr[2], vy.data()))) return rc;
+    if ((rc = upload_field(h, h->macr[3], vz.data()))) return rc;
+    if ((rc = upload_field(h, h->uold[0], vx.data()))) return rc;
+    if ((rc = upload_field(h, h->uold[1], vy.data()))) return rc;
+    if ((rc = upload_field(h, h->uold[2], vz.data()))) return rc;
+    std::vector<double> zero(n, 0.0);
+    if (h->macr[4] && (rc = upload_field(h, h->macr[4], zero.data()))) return rc;
+    // power-law omega starts at 0 (src/CUDA/lbmInitialization.cu:110-112)
+    if (h->omega_state && (rc = upload_field(h, h->omega_state, zero.data()))) return rc;
+    h->direct = true;
+    h->steps = 0;
+    return VLBM_OK;
+}
+
+int vlbm_set_omega(vlbm_h h, const double* omega) {
+    if (!h || !omega) return fail(VLBM_ERR_INVALID, "handle or omega is NULL");
+    if (!h->omega_state) return fail(VLBM_ERR_INVALID, "omega state exists for the power law only");
+    return upload_field(h, h->omega_state, omega);
+}
+
+int vlbm_load_pops(vlbm_h h, const double* pop) {
+    if (!h || !pop) return fail(VLBM_ERR_INVALID, "handle or populations NULL");
+    const vlbm_desc& d = h->d;
+    const size_t plane = (size_t)d.nx * d.ny;
+    std::vector<double> dev_in((size_t)d.q * h->n);
+    for (int i = 0; i < d.q; i++) {
+        const double w = pop_weight(d.q, i);
+        const double* srcp = pop + (size_t)i * plane * (d.nz + 1);   // skip the ghost plane
+        double* dstp = dev_in.data() + (size_t)i * h->n;
+        for (size_t k = 0; k < h->n; k++) dstp[k] = srcp[k] - w;
+    }
+    double* dbl = nullptr;
+    HIP_TRY(hipMalloc((void**)&dbl, dev_in.size() * 8));
+    int rc = VLBM_OK;
+    if (hipMemcpyAsync(dbl, dev_in.data(), dev_in.size() * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = fail(VLBM_ERR_DEVICE, "population upload failed");
+    else if (launch_convert(h->sbytes, h->lat[0], dbl, dev_in.size(), 0, h->stream))
+        rc = check_launch(-1, "conversion kernel");
+    else if (hipStreamSynchronize(h->stream) != hipSuccess)
+        rc = fail(VLBM_ERR_DEVICE, "sync failed");
+    (void)hipFree(dbl);
+    if (rc) return rc;
+    h->cur = 0;
+    h->direct = true;
+    h->steps = 0;
+    return VLBM_OK;
+}
+
+int vlbm_get_pops(vlbm_h h, double* pop) {
+    if (!h || !pop) return fail(VLBM_ERR_INVALID, "handle or output NULL");
+    const vlbm_desc& d = h->d;
+    void* source = h->lat[h->cur];
+    if (!h->direct) {
+        // post-stream + BC populations of the current state into the idle lattice
+        const int other = 1 - h->cur;
+        int rc = with_params(h, [&](void* pv, auto& p) {
+            p.src = h->lat[h->cur];
+            p.dst = h->lat[other];
+            p.aux_mode = kAuxMaterialize;
+            p.direct = 0;
+            return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream), "materialize kernel");
+        });
+        if (rc) return rc;
+        source = h->lat[other];
+    }
+    const size_t total = (size_t)d.q * h->n;
+    double* dbl = nullptr;
+    HIP_TRY(hipMalloc((void**)&dbl, total * 8));
+    std::vector<double> host(total);
+    int rc = VLBM_OK;
+    if (launch_convert(h->sbytes, source, dbl, total, 1, h->stream)) rc = check_launch(-1, "conversion kernel");
+    else if (hipMemcpyAsync(host.data(), dbl, total * 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+             hipStreamSynchronize(h->stream) != hipSuccess)
+        rc = fail(VLBM_ERR_DEVICE, "population download failed");
+    (void)hipFree(dbl);
+    if (rc) return rc;
+    const size_t plane = (size_t)d.nx * d.ny;
+    for (int i = 0; i < d.q; i++) {
+        const double w = pop_weight(d.q, i);
+        double* dstp = pop + (size_t)i * plane * (d.nz + 1);
+        const double* srcp = host.data() + (size_t)i * h->n;
+        for (size_t k = 0; k < h->n; k++) dstp[k] = srcp[k] + w;
+        std::memset(dstp + h->n, 0, plane * sizeof(double));
+    }
+    return VLBM_OK;
+}
+
+static int enqueue_steps(vlbm_h h, int n_steps, int save_last) {
+    const vlbm_desc& d = h->d;
+    for (int s = 0; s < n_steps; s++) {
+        const int other = 1 - h->cur;
+        const bool save = save_last && (s == n_steps - 1);
+        int rc = with_params(h, [&](void* pv, auto& p) {
+            p.src = h->lat[h->cur];
+            p.dst = h->lat[other];
+            p.save = save ? 1 : 0;
+            return check_launch(launch_step(d.q, h->sbytes, d.rheology, h->mask_geom, h->direct, pv, h->stream),
+                                "step kernel");
+        });
+        if (rc) return rc;
+        h->cur = other;
+        h->direct = false;
+        h->steps++;
+    }
+    return VLBM_OK;
+}
+
+int vlbm_step(vlbm_h h, int n_steps, int save_last) {
+    if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
+    if (n_steps < 0) return fail(VLBM_ERR_INVALID, "n_steps must be >= 0");
+    return enqueue_steps(h, n_steps, save_last);
+}
+
+int vlbm_time_steps(vlbm_h h, int n_steps, double* ms) {
+    if (!h || !ms) return fail(VLBM_ERR_INVALID, "handle or output NULL");
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    int rc = enqueue_steps(h, n_steps, 0);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    *ms = t;
+    return VLBM_OK;
+}
+
+int vlbm_compute_macroscopics(vlbm_h h) {
+    if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
+    const vlbm_desc& d = h->d;
+    return with_params(h, [&](void* pv, auto& p) {
+        p.src = h->lat[h->cur];
+        p.dst = h->lat[1 - h->cur];
+        p.aux_mode = kAuxMoments;
+        p.direct = h->direct ? 1 : 0;
+        return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream), "moments kernel");
+    });
+}
+
+int vlbm_macroscopics(vlbm_h h, double* rho, double* ux, double* uy, double* uz, double* omega) {
+    if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
+    double* outs[5] = {rho, ux, uy, uz, omega};
+    for (int k = 0; k < 5; k++) {
+        if (!outs[k]) continue;
+        if (!h->macr[k]) return fail(VLBM_ERR_INVALID, "omega exists for non-Newtonian fluids only");
+        int rc = fetch_field(h, h->macr[k], outs[k]);
+        if (rc) return rc;
+    }
+    return VLBM_OK;
+}
+
+int vlbm_reduce(vlbm_h h, double* residual, double* mean_rho) {
+    if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
+    if (launch_reduce(h->tbytes, h->macr[0], h->macr[1], h->macr[2], h->macr[3], h->uold[0], h->uold[1],
+                      h->uold[2], (uint32_t)h->n, h->partials, h->out3, h->stream))
+        return check_launch(-1, "reduction kernel");
+    double s[3];
+    HIP_TRY(hipMemcpyAsync(s, h->out3, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (residual) *residual = (s[1] != 0) ? s[0] / s[1] : 1.0;
+    if (mean_rho) *mean_rho = s[2] / (double)h->n;
+    return VLBM_OK;
+}
+
+// getVarFilename (src/CUDA/lbmReport.cpp:55-83): step zero-padded to at least 6 digits.
+static std::string var_filename(const char* path, const char* id, const char* var, unsigned step) {
+    unsigned n_zeros = 0, pot10 = 10;
+    const unsigned aux1 = 1000000;
+    if (step != 0)
+        for (n_zeros = 0; (unsigned long long)step * pot10 < aux1; pot10 *= 10) n_zeros++;
+    else
+        n_zeros = 6;
+    std::string s = std::string(path) + "/" + id + "/" + id + "_" + var;
+    for (unsigned i = 0; i < n_zeros; i++) s += "0";
+    return s + std::to_string(step) + ".bin";
+}
+
+int vlbm_save_macr_bin(vlbm_h h, const char* path, const char* id, unsigned step) {
+    if (!h || !path || !id) return fail(VLBM_ERR_INVALID, "NULL argument");
+    const char* names[5] = {"rho", "ux", "uy", "uz", "omega"};
+    const int nfields = h->d.rheology == VLBM_NEWTONIAN ? 4 : 5;
+    std::vector<double> buf(h->n);
+    for (int k = 0; k < nfields; k++) {
+        int rc = fetch_field(h, h->macr[k], buf.data());
+        if (rc) return rc;
+        const std::string fn = var_filename(path, id, names[k], step);
+        FILE* f = std::fopen(fn.c_str(), "wb");
+        if (!f) return fail(VLBM_ERR_IO, "cannot open " + fn);
+        size_t wrote;
+        if (h->d.precision == VLBM_FP64) {
+            wrote = std::fwrite(buf.data(), 8, h->n, f);
+        } else {
+            std::vector<float> fl(h->n);
+            for (size_t i = 0; i < h->n; i++) fl[i] = (float)buf[i];
+            wrote = std::fwrite(fl.data(), 4, h->n, f);
+        }
+        std::fclose(f);
+        if (wrote != h->n) return fail(VLBM_ERR_IO, "short write to " + fn);
+    }
+    return VLBM_OK;
+}
+
+int vlbm_save_info(vlbm_h h, const char* path, const char* id, unsigned total_steps, double mlups, double seconds) {
+    if (!h || !path || !id) return fail(VLBM_ERR_INVALID, "NULL argument");
+    const vlbm_desc& d = h->d;
+    const std::string fn = std::string(path) + "/" + id + "/" + id + "_info.txt";
+    FILE* f = std::fopen(fn.c_str(), "w");
+    if (!f) return fail(VLBM_ERR_IO, "cannot open " + fn);
+    const double pop_bytes = (double)d.q * d.nx * d.ny * (d.nz + 1) * (d.precision == VLBM_FP64 ? 8 : 4);
+    const double bw = seconds > 0 ? pop_bytes * 2.0 / (seconds * (double)(1u << 30)) * total_steps : 0.0;
+    std::fprintf(f, "---------------------------- SIMULATION INFORMATION ----------------------------\n");
+    std::fprintf(f, "      Simulation ID: %s\n", id);
+    std::fprintf(f, "       Velocity set: D3Q%d\n", d.q);
+    std::fprintf(f, "          Precision: %s\n", d.precision == VLBM_FP64 ? "double" : "float");
+    std::fprintf(f, "                 NX: %d\n", d.nx);
+    std::fprintf(f, "                 NY: %d\n", d.ny);
+    std::fprintf(f, "                 NZ: %d\n", d.nz);
+    std::fprintf(f, "           NZ_TOTAL: %d\n", d.nz);
+    std::fprintf(f, "                Tau: %.6e\n", d.tau);
+    std::fprintf(f, "               Umax: %.6e\n", d.ux_bc[1]);
+    std::fprintf(f, "                 FX: %.6e\n", d.fx);
+    std::fprintf(f, "                 FY: %.6e\n", d.fy);
+    std::fprintf(f, "                 FZ: %.6e\n", d.fz);
+    std::fprintf(f, "             Nsteps: %u\n", total_steps);
+    std::fprintf(f, "              MLUPS: %.1f\n", mlups);
+    std::fprintf(f, "          Bandwidht: %.1f (Gb/s)\n", bw);
+    std::fprintf(f, "       Time elapsed: %.3f (s)\n", seconds);
+    std::fprintf(f, "--------------------------------------------------------------------------------\n");
+    if (d.rheology != VLBM_NEWTONIAN) {
+        std::fprintf(f, "\n------------------------------ NON NEWTONIAN FLUID -----------------------------\n");
+        if (d.rheology == VLBM_POWERLAW) {
+            std::fprintf(f, "              Model: Power-Law\n");
+            std::fprintf(f, "        Power index: %.6e\n", d.n_index);
+            std::fprintf(f, " Consistency factor: %.6e\n", d.k_consistency);
+            std::fprintf(f, "            Gamma 0: %.6e\n", d.gamma_0);
+        } else {
+            std::fprintf(f, "              Model: Bingham\n");
+            std::fprintf(f, "  Plastic viscosity: %.6e\n", d.eta_p);
+            std::fprintf(f, "       Yield stress: %.6e\n", d.s_y);
+            std::fprintf(f, "      Plastic omega: %.6e\n", 1.0 / (3 * d.eta_p + 0.5));
+        }
+        std::fprintf(f, "--------------------------------------------------------------------------------\n");
+    }
+    const bool ok = std::ferror(f) == 0;
+    std::fclose(f);
+    if (!ok) return fail(VLBM_ERR_IO, "write error on " + fn);
+    return VLBM_OK;
+}
+
+int vlbm_steps_done(vlbm_h h, long long* steps) {
+    if (!h || !steps) return fail(VLBM_ERR_INVALID, "NULL argument");
+    *steps = h->steps;
+    return VLBM_OK;
+}
+
+int vlbm_sync(vlbm_h h) {
+    if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return VLBM_OK;
+}
+
+int vlbm_device_bytes(vlbm_h h, size_t* bytes) {
+    if (!h || !bytes) return fail(VLBM_ERR_INVALID, "NULL argument");
+    *bytes = h->device_bytes;
+    return VLBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,614 @@
+// HIP kernels of the LBM time step for gfx950 (templates; instantiated per
+// storage precision in kernels_fp64.hip, kernels_fp32.hip, kernels_fp16.hip).
+//
+// Design (replaces src/CUDA/lbm.cu:3-504 gpuMacrCollisionStream, lbm.cu:600-657
+// gpuPopulationsTransfer, lbm.cu:579-598 gpuApplyBC and the pointer swap of
+// src/CUDA/structs/populations.h:64-70 with ONE kernel per step):
+//
+//  * State = post-collision populations stored as deviations g_i = f_i - w_i,
+//    SoA planes [i][z][y][x], two lattices A/B (read one, write the other).
+//  * Pull streaming: node x gathers g_i from x - c_i with periodic wrap in all
+//    three directions. This equals the reference's push + ghost-plane fold, so
+//    the gathered values are the reference's post-stream populations.
+//  * Boundary conditions are applied in registers right after the gather. They
+//    need post-collision values at the node (bounce-back) or its tangential
+//    neighbours (free slip), which are exactly what the source lattice holds.
+//  * Then moments, apparent viscosity, regularized collision with Guo forcing,
+//    and one coalesced store per population.
+//
+// Every element of the source lattice is read by exactly one node, so the step
+// is a streaming kernel: 2*Q*sizeof(storage) bytes per node, HBM bound.
+
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include "lattice.h"
+
+namespace vlbm {
+
+template <int Q> struct BBTable { uint32_t m[26]; };
+template <int Q>
+constexpr BBTable<Q> make_bb_table() {
+    BBTable<Q> t{};
+    for (int d = 0; d < 26; d++) t.m[d] = bb_mask<Q>(d);
+    return t;
+}
+static __constant__ BBTable<19> c_bb19 = make_bb_table<19>();
+static __constant__ BBTable<27> c_bb27 = make_bb_table<27>();
+
+// ---- storage conversions ------------------------------------------------------
+template <typename S, typename T> __device__ __forceinline__ T ld(const S* p) { return (T)(*p); }
+template <> __device__ __forceinline__ float ld<_Float16, float>(const _Float16* p) { return (float)(*p); }
+
+template <typename S, typename T> __device__ __forceinline__ void st_nt(S* p, T v) {
+    __builtin_nontemporal_store((S)v, p);
+}
+
+template <typename S> struct ComputeOf { using type = float; };
+template <> struct ComputeOf<double> { using type = double; };
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    return (uint32_t)(((uint64_t)n * f.m) >> f.shift);
+}
+
+// Box-builder region of a node, in the reference builders' if-chain priority
+// (corners, then edges, then faces; e.g. src/CUDA/boundaryConditionsBuilders/closedBoxBounceBack.cu:33-186).
+__device__ __forceinline__ int box_region(int x, int y, int z, int nx, int ny, int nz) {
+    const bool s = (y == 0), n = (y == ny - 1), w = (x == 0), e = (x == nx - 1);
+    const bool b = (z == 0), f = (z == nz - 1);
+    if (!(s | n | w | e | b | f)) return 26;
+    if (s && w && b) return 0;
+    if (s && w && f) return 1;
+    if (s && e && b) return 2;
+    if (s && e && f) return 3;
+    if (n && w && b) return 4;
+    if (n && w && f) return 5;
+    if (n && e && b) return 6;
+    if (n && e && f) return 7;
+    if (s && w) return 8;
+    if (s && e) return 9;
+    if (n && w) return 10;
+    if (n && e) return 11;
+    if (s && b) return 12;
+    if (s && f) return 13;
+    if (n && b) return 14;
+    if (n && f) return 15;
+    if (w && b) return 16;
+    if (w && f) return 17;
+    if (e && b) return 18;
+    if (e && f) return 19;
+    if (s) return 20;
+    if (n) return 21;
+    if (w) return 22;
+    if (e) return 23;
+    if (b) return 24;
+    return 25;
+}
+
+template <typename T>
+struct Node {
+    int x, y, z;
+    int xm, xp, ym, yp, zm, zp;     // periodic neighbours
+    uint32_t idx;
+};
+
+template <typename T>
+__device__ __forceinline__ uint32_t row_off(const Params<T>& p, int ys, int zs) {
+    return ((uint32_t)zs * (uint32_t)p.ny + (uint32_t)ys) * (uint32_t)p.nx;
+}
+
+// Source node of population i for the node (pull streaming with wrap).
+template <typename T>
+__device__ __forceinline__ uint32_t src_index(const Params<T>& p, const Node<T>& nd, int i) {
+    const int xs = kCX[i] > 0 ? nd.xm : (kCX[i] < 0 ? nd.xp : nd.x);
+    const int ys = kCY[i] > 0 ? nd.ym : (kCY[i] < 0 ? nd.yp : nd.y);
+    const int zs = kCZ[i] > 0 ? nd.zm : (kCZ[i] < 0 ? nd.zp : nd.z);
+    return row_off(p, ys, zs) + (uint32_t)xs;
+}
+
+template <typename S, typename T>
+__device__ __forceinline__ T load_pop(const Params<T>& p, int i, uint32_t node) {
+    const S* base = (const S*)p.src;
+    return ld<S, T>(base + (int64_t)i * p.stride + node);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions on gathered deviations g (post-stream values).
+// ---------------------------------------------------------------------------
+
+// Zou-He on a face (D3Q19), generic form: with outward normal n = s*e_a,
+// T = sum of populations with c_a = 0, K = sum of outgoing ones (c.n > 0),
+//   velocity: rho_w = (T + 2K) / (1 + u.n)
+//   pressure: u.n   = (T + 2K) / rho_w - 1, tangential velocity 0
+//   incoming i:  f_i = f_opp(i) + 6 w_i rho_w (c_i.u) - sum_t c_it N_t,
+//   N_t = 1/2 sum_{c_a=0} f_j c_jt - rho_w u_t / 3.
+// Reference-compatible lines (src/CUDA/boundaryConditionsSchemes/D3Q19_VelZouHe.cu,
+// D3Q19_PresZouHe.cu), used unless zouhe_textbook is set:
+//   * velocity BACK uses 1/(1+uz_w) (VelZouHe.cu:153) and u_x in place of u_y
+//     for populations 11 and 18 (VelZouHe.cu:168-171);
+//   * velocity WEST population 15 uses u_y in place of u_z (VelZouHe.cu:95);
+//   * EAST (velocity and pressure) adds to each incoming population its own
+//     streamed value the WEST increment of its opposite (VelZouHe.cu:115-119,
+//     PresZouHe.cu:115-119).
+// Zou-He works on deviations unchanged: the weights of T + 2K sum to 1 and
+// the tangential sums are antisymmetric.
+template <typename T>
+__device__ __forceinline__ void zou_he(T* g, int dir, bool pressure, T uxw, T uyw, T uzw, T rhow_in,
+                                       bool compat) {
+    if (dir > dB) return;
+    const int a = (dir == dN || dir == dS) ? 1 : ((dir == dW || dir == dE) ? 0 : 2);
+    const int s = (dir == dN || dir == dE || dir == dF) ? 1 : -1;
+    auto ca = [&](int i) -> int { return a == 0 ? kCX[i] : (a == 1 ? kCY[i] : kCZ[i]); };
+    T tsum = 0, ksum = 0, n_t[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        const int c = ca(i);
+        if (c == 0) {
+            tsum += g[i];
+            n_t[0] += (T)kCX[i] * g[i];
+            n_t[1] += (T)kCY[i] * g[i];
+            n_t[2] += (T)kCZ[i] * g[i];
+        } else if (c == s) {
+            ksum += g[i];
+        }
+    }
+    const T tk = tsum + 2 * ksum;     // (T + 2K) - 1 in deviations
+    T u[3] = {uxw, uyw, uzw};
+    T rho_w;
+    if (pressure) {
+        rho_w = rhow_in;
+        const T un = (tk + ((T)1 - rho_w)) / rho_w;   // u.n
+        u[0] = u[1] = u[2] = 0;
+        u[a] = (T)s * un;
+    } else {
+        T denom = (T)1 + (T)s * u[a];
+        if (compat && dir == dB) denom = (T)1 + uzw;
+        rho_w = ((T)1 + tk) / denom;
+    }
+    T nt[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) nt[t] = (T)0.5 * n_t[t] - (pressure ? (T)0 : rho_w * u[t] / (T)3);
+    nt[a] = 0;
+    const bool east_compat = compat && dir == dE;
+    T out[19];
+#pragma unroll
+    for (int i = 1; i < 19; i++) out[i] = g[i];
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        if (ca(i) != -s) continue;                  // incoming populations only
+        const int j = east_compat ? opp(i) : i;     // population whose formula is used
+        T cu;
+        if (compat && !pressure && dir == dB && (j == 11 || j == 18))
+            cu = (T)kCZ[j] * u[2] + (T)kCY[j] * u[0];
+        else if (compat && !pressure && (dir == dW || dir == dE) && j == 15)
+            cu = (T)kCX[j] * u[0] + (T)kCZ[j] * u[1];
+        else
+            cu = (T)kCX[j] * u[0] + (T)kCY[j] * u[1] + (T)kCZ[j] * u[2];
+        const T corr = (T)kCX[j] * nt[0] + (T)kCY[j] * nt[1] + (T)kCZ[j] * nt[2];
+        const T inc = (T)(6.0 * weight<19>(j)) * rho_w * cu - corr;
+        out[i] = (east_compat ? g[i] : g[opp(i)]) + inc;
+    }
+#pragma unroll
+    for (int i = 1; i < 19; i++) g[i] = out[i];
+}
+
+// Velocity bounce-back (src/CUDA/boundaryConditionsSchemes/D3Q19_VelBounceBack.cu:27-149):
+// f_i = f_opp(i) + 6 rho w_i (c_i . u_w), everything post-stream, rho summed
+// over all post-stream populations of the node.
+template <int Q, typename T>
+__device__ __forceinline__ void vel_bounce_back(T* g, int dir, T uxw, T uyw, T uzw) {
+    if (dir > dB) return;
+    T dr = 0;
+#pragma unroll
+    for (int i = 0; i < Q; i++) dr += g[i];
+    const T rho = (T)1 + dr;
+    const unsigned faces = kDirFaces[dir];
+    T out[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) out[i] = g[i];
+#pragma unroll
+    for (int i = 1; i < Q; i++) {
+        if (!incoming(i, faces)) continue;
+        const T cu = (T)kCX[i] * uxw + (T)kCY[i] * uyw + (T)kCZ[i] * uzw;
+        out[i] = g[opp(i)] + (T)(6.0 * weight<Q>(i)) * rho * cu;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; i++) g[i] = out[i];
+}
+
+// Interpolated bounce-back (Bouzidi, linear) against the duct circle(s)
+// (src/CUDA/boundaryConditionsSchemes/interpolatedBounceBack.cu:6-431).
+// q is the normalised distance node -> wall along the D2Q9 projection of the
+// incoming population, evaluated with the reference's distPoints2D (first
+// difference cast to float, src/CUDA/globalFunctions.h:158-162).
+template <typename T>
+__device__ __forceinline__ T dist2d_ref(T x1, T y1, T x2, T y2) {
+    return (T)sqrt((T)((float)(x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2)));
+}
+template <typename T>
+__device__ T wall_q(int kind, T x1, T y1, T x2, T y2, T R) {
+    if (kind == 0) {           // along x
+        T y = y2, x = sqrt(R * R - y * y);
+        if (x2 < 0) x = -x;
+        return dist2d_ref(x, y, x1, y1);
+    } else if (kind == 1) {    // along y
+        T x = x2, y = sqrt(R * R - x * x);
+        if (y2 < 0) y = -y;
+        return dist2d_ref(x, y, x1, y1);
+    } else if (kind == 2) {    // along the (1,1) diagonal
+        const T b = y1 - x1, dr = sqrt(2 * R * R - b * b);
+        T x = (-b + dr) * (T)0.5;
+        if ((x1 > x2 && (x > x1 || x < x2)) || (x1 < x2 && (x < x1 || x > x2))) x = (-b - dr) * (T)0.5;
+        return dist2d_ref(x, x + b, x1, y1) / (T)1.41421356237309504880;
+    } else {                   // along the (1,-1) diagonal
+        const T b = y1 + x1, dr = sqrt(2 * R * R - b * b);
+        T x = (b + dr) * (T)0.5;
+        if ((x1 > x2 && (x > x1 || x < x2)) || (x1 < x2 && (x < x1 || x > x2))) x = (b - dr) * (T)0.5;
+        return dist2d_ref(x, -x + b, x1, y1) / (T)1.41421356237309504880;
+    }
+}
+
+template <int Q, typename S, typename T>
+__device__ void interp_bounce_back(T* g, const Params<T>& p, const Node<T>& nd, unsigned unknown, bool inside) {
+    // D2Q9 projections in the reference's bit order (nodeTypeMap.h:100-107)
+    const int dxs[8] = {1, 0, -1, 0, 1, -1, -1, 1};
+    const int dys[8] = {0, 1, 0, -1, 1, 1, -1, -1};
+    const int kinds[8] = {0, 1, 0, 1, 2, 3, 2, 3};
+    const T xN = (T)nd.x - (T)(p.nx - 1) / (T)2, yN = (T)nd.y - (T)(p.ny - 1) / (T)2;
+    const T rr = sqrt(xN * xN + yN * yN);
+    const T c = xN / rr, s = yN / rr;
+    T radius, uxw, uyw, uzw;
+    if (inside) {
+        radius = p.duct_r; uxw = -p.w_in * radius * s; uyw = p.w_in * radius * c; uzw = p.u_in;
+    } else {
+        const T wo = p.w_out * ((T)nd.z / (T)p.nz);
+        radius = p.duct_R; uxw = -wo * radius * s; uyw = wo * radius * c; uzw = p.u_out;
+    }
+    const uint32_t self = nd.idx;
+    for (int b = 0; b < 8; b++) {
+        if (!(unknown & (1u << b))) continue;
+        const T qd = wall_q<T>(kinds[b], xN, yN, xN - (T)dxs[b], yN - (T)dys[b], radius);
+#pragma unroll
+        for (int i = 1; i < Q; i++) {
+            if (kCX[i] != dxs[b] || kCY[i] != dys[b]) continue;
+            const int j = opp(i);
+            const T cu = (T)kCX[i] * uxw + (T)kCY[i] * uyw + (T)kCZ[i] * uzw;
+            const T df = (T)weight<Q>(i) * cu;
+            const T fj = load_pop<S, T>(p, j, self);
+            if (qd > (T)0.5) {
+                const T fi = load_pop<S, T>(p, i, self);
+                g[i] = ((T)0.5 / qd) * fj + ((qd - (T)0.5) / qd) * fi + (T)3 * df / qd;
+            } else {
+                const int xa = nd.x + kCX[i], ya = nd.y + kCY[i];
+                const int za = kCZ[i] > 0 ? nd.zp : (kCZ[i] < 0 ? nd.zm : nd.z);
+                const T fa = load_pop<S, T>(p, j, row_off(p, ya, za) + (uint32_t)xa);
+                g[i] = 2 * qd * fj + (1 - 2 * qd) * fa + (T)6 * df;
+            }
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Per-node pieces
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ Node<T> make_node(const Params<T>& p, uint32_t idx) {
+    Node<T> nd;
+    const uint32_t yz = fdiv(idx, p.div_nx);
+    nd.x = (int)(idx - yz * (uint32_t)p.nx);
+    nd.z = (int)fdiv(idx, p.div_nxy);
+    nd.y = (int)(yz - (uint32_t)nd.z * (uint32_t)p.ny);
+    nd.idx = idx;
+    nd.xm = nd.x == 0 ? p.nx - 1 : nd.x - 1;
+    nd.xp = nd.x == p.nx - 1 ? 0 : nd.x + 1;
+    nd.ym = nd.y == 0 ? p.ny - 1 : nd.y - 1;
+    nd.yp = nd.y == p.ny - 1 ? 0 : nd.y + 1;
+    nd.zm = nd.z == 0 ? p.nz - 1 : nd.z - 1;
+    nd.zp = nd.z == p.nz - 1 ? 0 : nd.z + 1;
+    return nd;
+}
+
+template <bool MASK, typename T>
+__device__ __forceinline__ uint32_t node_word(const Params<T>& p, const Node<T>& nd) {
+    if (MASK) return p.mask[nd.idx];
+    return p.region_word[box_region(nd.x, nd.y, nd.z, p.nx, p.ny, p.nz)];
+}
+
+// Gather (pull streaming) of the post-stream deviations.
+template <int Q, typename S, typename T>
+__device__ __forceinline__ void gather(T* g, const Params<T>& p, const Node<T>& nd) {
+#pragma unroll
+    for (int i = 0; i < Q; i++) g[i] = load_pop<S, T>(p, i, src_index(p, nd, i));
+}
+
+template <int Q, typename S, typename T>
+__device__ __forceinline__ void load_direct(T* g, const Params<T>& p, const Node<T>& nd) {
+#pragma unroll
+    for (int i = 0; i < Q; i++) g[i] = load_pop<S, T>(p, i, nd.idx);
+}
+
+// Boundary condition of the node's word on the gathered populations
+// (dispatch of src/CUDA/boundaryConditionsHandler.cu:25-79).
+template <int Q, typename S, typename T>
+__device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>& nd, uint32_t word) {
+    const int scheme = w_scheme(word);
+    const int dir = w_dir(word);
+    if (scheme == kBounceBack) {
+        // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
+        if (dir >= dNW && w_geom(word)) return;
+        if (dir > dSEB) return;
+        const uint32_t m = (Q == 19) ? c_bb19.m[dir] : c_bb27.m[dir];
+#pragma unroll
+        for (int i = 1; i < Q; i++)
+            if (m & (1u << i)) g[i] = load_pop<S, T>(p, opp(i), nd.idx);
+    } else if (scheme == kFreeSlip) {
+        // specular reflection: incoming i comes from the mirror population m
+        // of the node displaced by minus the tangential part of c_i
+        // (src/CUDA/boundaryConditionsSchemes/freeSlip.cu:26-168)
+        if (dir > dB) return;
+        const unsigned faces = kDirFaces[dir];
+#pragma unroll
+        for (int i = 1; i < Q; i++) {
+            if (!incoming(i, faces)) continue;
+            int mx = kCX[i], my = kCY[i], mz = kCZ[i];
+            int sx = nd.x, sy = nd.y, sz = nd.z;
+            if (dir == dN || dir == dS) {
+                my = -my;
+                sx = mx > 0 ? nd.xm : (mx < 0 ? nd.xp : nd.x);
+                sz = mz > 0 ? nd.zm : (mz < 0 ? nd.zp : nd.z);
+            } else if (dir == dW || dir == dE) {
+                mx = -mx;
+                sy = my > 0 ? nd.ym : (my < 0 ? nd.yp : nd.y);
+                sz = mz > 0 ? nd.zm : (mz < 0 ? nd.zp : nd.z);
+            } else {
+                mz = -mz;
+                sx = mx > 0 ? nd.xm : (mx < 0 ? nd.xp : nd.x);
+                sy = my > 0 ? nd.ym : (my < 0 ? nd.yp : nd.y);
+            }
+            const int m = find_pop(mx, my, mz);
+            g[i] = load_pop<S, T>(p, m, row_off(p, sy, sz) + (uint32_t)sx);
+        }
+    } else if (scheme == kVelBB) {
+        if (Q == 19)
+            vel_bounce_back<Q, T>(g, dir, p.ux_bc[w_uxi(word)], p.uy_bc[w_uyi(word)], p.uz_bc[w_uzi(word)]);
+    } else if (scheme == kVelZouHe || scheme == kPresZouHe) {
+        if (Q == 19)
+            zou_he<T>(g, dir, scheme == kPresZouHe, p.ux_bc[w_uxi(word)], p.uy_bc[w_uyi(word)],
+                      p.uz_bc[w_uzi(word)], p.rho_bc[w_rhoi(word)], !p.zouhe_textbook);
+    } else if (scheme == kInterpBB) {
+        interp_bounce_back<Q, S, T>(g, p, nd, w_interp(word), dir == dS);
+    }
+}
+
+// Moments and apparent relaxation frequency of the (post-BC) deviations.
+template <int Q, int RHEO, typename T>
+struct Moments {
+    T dr, rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, omega;
+};
+
+// omega_old: power-law state of the node (ignored otherwise).
+template <int Q, int RHEO, typename T>
+__device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<T>& p, T omega_old) {
+    Moments<Q, RHEO, T> m;
+    T dr = 0, jx = 0, jy = 0, jz = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const T v = g[i];
+        dr += v;
+        if (kCX[i]) { jx += (T)kCX[i] * v; sxx += v; }
+        if (kCY[i]) { jy += (T)kCY[i] * v; syy += v; }
+        if (kCZ[i]) { jz += (T)kCZ[i] * v; szz += v; }
+        if (kCX[i] * kCY[i]) sxy += (T)(kCX[i] * kCY[i]) * v;
+        if (kCX[i] * kCZ[i]) sxz += (T)(kCX[i] * kCZ[i]) * v;
+        if (kCY[i] * kCZ[i]) syz += (T)(kCY[i] * kCZ[i]) * v;
+    }
+    const T rho = (T)1 + dr;
+    const T inv = (T)1 / rho;
+    const T ux = (jx + (T)0.5 * p.fx) * inv;
+    const T uy = (jy + (T)0.5 * p.fy) * inv;
+    const T uz = (jz + (T)0.5 * p.fz) * inv;
+    // Pi^neq = sum f c c - rho (u u + I/3) = sum g c c - rho u u - (rho-1)/3 I
+    const T dr3 = dr / (T)3;
+    m.pxx = sxx - rho * ux * ux - dr3;
+    m.pyy = syy - rho * uy * uy - dr3;
+    m.pzz = szz - rho * uz * uz - dr3;
+    m.pxy = sxy - rho * ux * uy;
+    m.pxz = sxz - rho * ux * uz;
+    m.pyz = syz - rho * uy * uz;
+    m.dr = dr; m.rho = rho; m.ux = ux; m.uy = uy; m.uz = uz;
+    T om = p.omega;
+    if (RHEO != 0) {
+        // |Pi^neq + sym(u F)| (src/CUDA/lbm.cu:192-208)
+        const T axx = m.pxx + ux * p.fx, ayy = m.pyy + uy * p.fy, azz = m.pzz + uz * p.fz;
+        const T axy = m.pxy + (ux * p.fy + uy * p.fx) * (T)0.5;
+        const T axz = m.pxz + (ux * p.fz + uz * p.fx) * (T)0.5;
+        const T ayz = m.pyz + (uy * p.fz + uz * p.fy) * (T)0.5;
+        const T smag = sqrt((T)0.5 * (axx * axx + ayy * ayy + azz * azz + 2 * (axy * axy + axz * axz + ayz * ayz)));
+        if (RHEO == 1) {
+            // Bingham: omega = omega_p max(0, 1 - tau_y/|S|) (src/CUDA/NNF/nnf.h:83);
+            // a zero yield stress is the Newtonian plastic fluid.
+            if (p.s_y == (T)0) {
+                om = p.omega_p;
+            } else {
+                const T t = (T)1 - p.s_y / smag;
+                om = p.omega_p * ((T)0 > t ? (T)0 : t);
+            }
+        } else {
+            // power law with the previous step's omega (src/CUDA/NNF/nnf.h:38-53)
+            T eta = ((T)1 / omega_old - (T)0.5) / (T)3;
+            const T gdot = ((T)1 - (T)0.5 * omega_old) * smag / eta;
+            eta = (gdot <= p.gamma_0) ? p.k_cons : p.k_cons * pow(gdot, p.n_minus_1);
+            om = (T)1 / ((T)0.5 + (T)3 * eta);
+        }
+        if (p.clamp) {
+            // tau in [tau_min, tau_max]  <=>  omega in [omega_lo, omega_hi]
+            if (!(om <= p.omega_hi)) om = p.omega_hi;
+            else if (om < p.omega_lo) om = p.omega_lo;
+        }
+    }
+    m.omega = om;
+    return m;
+}
+
+// Regularized collision with Guo forcing, in deviation form:
+//   g*_i = w_i [ (rho-1) + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u.u) ]
+//        + (1-omega) 4.5 w_i (Q_i : Pi^neq - c_i.F/3)
+//        + (1-omega/2) w_i (3 c.F + 9 (c.u)(c.F) - 3 u.F)
+// with Q_i = c_i c_i - I/3 (src/CUDA/lbm.cu:263-461).
+template <int Q, int RHEO, typename T>
+__device__ __forceinline__ void collide(T* g, const Moments<Q, RHEO, T>& m, const Params<T>& p) {
+    const T om = m.omega;
+    const T a = (T)4.5 * ((T)1 - om);
+    const T b = (T)1 - (T)0.5 * om;
+    const T uf = m.ux * p.fx + m.uy * p.fy + m.uz * p.fz;
+    const T usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
+    const T tr = m.pxx + m.pyy + m.pzz;
+    const T k0 = m.dr - (T)1.5 * m.rho * usq - a * tr / (T)3 - (T)3 * b * uf;
+    const T r3 = (T)3 * m.rho, r45 = (T)4.5 * m.rho;
+    const T a3 = a / (T)3, b3 = (T)3 * b, b9 = (T)9 * b;
+    const T pxy2 = 2 * m.pxy, pxz2 = 2 * m.pxz, pyz2 = 2 * m.pyz;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+        const T cu = (T)cx * m.ux + (T)cy * m.uy + (T)cz * m.uz;
+        const T cf = (T)cx * p.fx + (T)cy * p.fy + (T)cz * p.fz;
+        T cpc = 0;
+        if (cx) cpc += m.pxx;
+        if (cy) cpc += m.pyy;
+        if (cz) cpc += m.pzz;
+        if (cx * cy) cpc += (T)(cx * cy) * pxy2;
+        if (cx * cz) cpc += (T)(cx * cz) * pxz2;
+        if (cy * cz) cpc += (T)(cy * cz) * pyz2;
+        const T v = k0 + cu * (r3 + r45 * cu + b9 * cf) + a * cpc + (b3 - a3) * cf;
+        g[i] = (T)weight<Q>(i) * v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 256;
+
+// One time step. DIRECT: the source holds post-stream populations (first step
+// after init / load), read in place without streaming or BC.
+template <int Q, typename S, int RHEO, bool MASK, bool DIRECT>
+__global__ __launch_bounds__(kBlock) void k_step(const Params<typename ComputeOf<S>::type> p) {
+    using T = typename ComputeOf<S>::type;
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= p.n) return;
+    const Node<T> nd = make_node(p, idx);
+    const uint32_t word = node_word<MASK>(p, nd);
+    if (!w_used(word)) return;
+    T g[Q];
+    if (DIRECT) {
+        load_direct<Q, S, T>(g, p, nd);
+    } else {
+        gather<Q, S, T>(g, p, nd);
+        if (w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
+    }
+    const T om_old = (RHEO == 2) ? p.omega_state[idx] : (T)0;
+    const Moments<Q, RHEO, T> m = moments<Q, RHEO, T>(g, p, om_old);
+    if (RHEO == 2) p.omega_state[idx] = m.omega;
+    if (p.save) {
+        p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
+        if (RHEO != 0) p.m_omega[idx] = m.omega;
+    }
+    collide<Q, RHEO, T>(g, m, p);
+    S* dst = (S*)p.dst;
+#pragma unroll
+    for (int i = 0; i < Q; i++) st_nt<S, T>(dst + (int64_t)i * p.stride + idx, g[i]);
+}
+
+// Auxiliary pass over the current state without advancing it:
+//   kAuxMoments:     macroscopics the next step would compute
+//   kAuxMaterialize: post-stream, post-BC populations into dst
+template <int Q, typename S, int RHEO, bool MASK>
+__global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<S>::type> p) {
+    using T = typename ComputeOf<S>::type;
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= p.n) return;
+    const Node<T> nd = make_node(p, idx);
+    const uint32_t word = node_word<MASK>(p, nd);
+    T g[Q];
+    if (p.direct || !w_used(word)) {
+        load_direct<Q, S, T>(g, p, nd);
+    } else {
+        gather<Q, S, T>(g, p, nd);
+        if (w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
+    }
+    if (p.aux_mode == kAuxMaterialize) {
+        S* dst = (S*)p.dst;
+#pragma unroll
+        for (int i = 0; i < Q; i++) dst[(int64_t)i * p.stride + idx] = (S)g[i];
+        return;
+    }
+    if (!w_used(word)) return;
+    const T om_old = (RHEO == 2) ? p.omega_state[idx] : (T)0;
+    const Moments<Q, RHEO, T> m = moments<Q, RHEO, T>(g, p, om_old);
+    p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
+    if (RHEO != 0) p.m_omega[idx] = m.omega;
+}
+
+// Equilibrium deviations from (rho, u) (gpu_f_eq, src/CUDA/globalFunctions.h:50-56).
+template <int Q, typename S>
+__global__ __launch_bounds__(kBlock) void k_fill_feq(S* state, int64_t stride, uint32_t n, const double* rho,
+                                                     const double* ux, const double* uy, const double* uz) {
+    using T = typename ComputeOf<S>::type;
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= n) return;
+    const T r = (T)rho[idx], vx = (T)ux[idx], vy = (T)uy[idx], vz = (T)uz[idx];
+    const T usq = vx * vx + vy * vy + vz * vz;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const T cu = (T)kCX[i] * vx + (T)kCY[i] * vy + (T)kCZ[i] * vz;
+        const T w = (T)weight<Q>(i);
+        // w (rho (1 - 1.5 u.u + 3 c.u + 4.5 (c.u)^2) - 1)
+        const T dev = w * ((r - (T)1) + r * ((T)3 * cu + (T)4.5 * cu * cu - (T)1.5 * usq));
+        state[(int64_t)i * stride + idx] = (S)dev;
+    }
+}
+
+// Residual / mean-density partial sums with FP64 accumulation: per block
+// {sum |u - u_old|, sum |u|, sum rho}; u_old <- u.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reduce_partial(const T* rho, const T* ux, const T* uy, const T* uz,
+                                                           T* oux, T* ouy, T* ouz, uint32_t n, double* partials) {
+    __shared__ double sh[3][kBlock / 64];
+    double a = 0, b = 0, c = 0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double vx = ux[i], vy = uy[i], vz = uz[i];
+        const double dx = vx - (double)oux[i], dy = vy - (double)ouy[i], dz = vz - (double)ouz[i];
+        a += sqrt(dx * dx + dy * dy + dz * dz);
+        b += sqrt(vx * vx + vy * vy + vz * vz);
+        c += (double)rho[i];
+        oux[i] = ux[i]; ouy[i] = uy[i]; ouz[i] = uz[i];
+    }
+    // wave64 tree
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { sh[0][wave] = a; sh[1][wave] = b; sh[2][wave] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int w = 0; w < kBlock / 64; w++) { s0 += sh[0][w]; s1 += sh[1][w]; s2 += sh[2][w]; }
+        partials[3 * blockIdx.x + 0] = s0;
+        partials[3 * blockIdx.x + 1] = s1;
+        partials[3 * blockIdx.x + 2] = s2;
+    }
+}
+
+// Element-wise conversion between a storage-typed buffer and doubles.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_convert(S* buf, double* dbl, size_t n, int to_double) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (to_double) dbl[i] = (double)buf[i];
+    else buf[i] = (S)dbl[i];
+}
+
+}  // namespace vlbm

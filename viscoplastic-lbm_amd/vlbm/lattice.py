@@ -1,0 +1,150 @@
+"""One lattice resident on the GPU, driven through the C-ABI (include/vlbm.h).
+
+Mirrors the reference's per-simulation objects (Populations, Macroscopics,
+NodeTypeMap; src/CUDA/structs/) and the operations main.cu applies to them.
+Arrays cross the boundary as numpy float64, shaped [NZ][NY][NX] for scalar
+fields and [Q][NZ+1][NY][NX] for populations (the reference's idxPop layout).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> Optional[np.ndarray]:
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def build_map(desc: _capi.VlbmDesc) -> np.ndarray:
+    """Node-type words [NZ][NY][NX] (host only; gpuBuildBoundaryConditions)."""
+    out = np.zeros((desc.nz, desc.ny, desc.nx), dtype=np.uint32)
+    check(lib.vlbm_build_map(C.byref(desc), out.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return out
+
+
+class Lattice:
+    def __init__(self, desc: _capi.VlbmDesc):
+        self.desc = desc
+        h = C.c_void_p()
+        check(lib.vlbm_create(C.byref(desc), C.byref(h)))
+        self._h = h
+        self.shape = (desc.nz, desc.ny, desc.nx)
+        self.q = desc.q
+
+    @classmethod
+    def from_config(cls, cfg) -> "Lattice":
+        return cls(cfg.to_desc())
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib.vlbm_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ----------------------------------------------------------------
+    def init(self, rho=None, ux=None, uy=None, uz=None) -> None:
+        arrs = [_f64(a) for a in (rho, ux, uy, uz)]
+        for a in arrs:
+            if a is not None and a.size != np.prod(self.shape):
+                raise ValueError("initial fields must have NX*NY*NZ entries")
+        check(lib.vlbm_init(self._h, *[_ptr(a) for a in arrs]))
+
+    def load_pops(self, pop: np.ndarray) -> None:
+        pop = _f64(pop)
+        if pop.size != self.q * (self.shape[0] + 1) * self.shape[1] * self.shape[2]:
+            raise ValueError("populations must have Q*(NZ+1)*NY*NX entries")
+        check(lib.vlbm_load_pops(self._h, _ptr(pop)))
+
+    def get_pops(self) -> np.ndarray:
+        nz, ny, nx = self.shape
+        out = np.zeros((self.q, nz + 1, ny, nx), dtype=np.float64)
+        check(lib.vlbm_get_pops(self._h, _ptr(out)))
+        return out
+
+    def set_omega(self, omega: np.ndarray) -> None:
+        check(lib.vlbm_set_omega(self._h, _ptr(_f64(omega))))
+
+    # -- stepping -------------------------------------------------------------
+    def step(self, n: int = 1, save_last: bool = False) -> None:
+        check(lib.vlbm_step(self._h, int(n), 1 if save_last else 0))
+
+    def time_steps(self, n: int) -> float:
+        ms = C.c_double()
+        check(lib.vlbm_time_steps(self._h, int(n), C.byref(ms)))
+        return ms.value
+
+    def sync(self) -> None:
+        check(lib.vlbm_sync(self._h))
+
+    @property
+    def steps_done(self) -> int:
+        s = C.c_longlong()
+        check(lib.vlbm_steps_done(self._h, C.byref(s)))
+        return s.value
+
+    @property
+    def device_bytes(self) -> int:
+        b = C.c_size_t()
+        check(lib.vlbm_device_bytes(self._h, C.byref(b)))
+        return b.value
+
+    # -- macroscopics -----------------------------------------------------------
+    def compute_macroscopics(self) -> None:
+        check(lib.vlbm_compute_macroscopics(self._h))
+
+    def macroscopics(self, omega: bool = False) -> dict:
+        out = {k: np.zeros(self.shape) for k in ("rho", "ux", "uy", "uz")}
+        om = np.zeros(self.shape) if omega else None
+        check(lib.vlbm_macroscopics(self._h, _ptr(out["rho"]), _ptr(out["ux"]), _ptr(out["uy"]),
+                                    _ptr(out["uz"]), _ptr(om)))
+        if omega:
+            out["omega"] = om
+        return out
+
+    def reduce(self) -> Tuple[float, float]:
+        res, mr = C.c_double(), C.c_double()
+        check(lib.vlbm_reduce(self._h, C.byref(res), C.byref(mr)))
+        return res.value, mr.value
+
+    def bc_nodes(self) -> np.ndarray:
+        cnt = C.c_size_t()
+        check(lib.vlbm_bc_nodes(self._h, None, 0, C.byref(cnt)))
+        out = np.zeros(cnt.value, dtype=np.uint64)
+        if cnt.value:
+            check(lib.vlbm_bc_nodes(self._h, out.ctypes.data_as(C.POINTER(C.c_uint64)), cnt.value, C.byref(cnt)))
+        return out
+
+    # -- output -----------------------------------------------------------------
+    def save_macr_bin(self, path: str, sim_id: str, step: int) -> None:
+        check(lib.vlbm_save_macr_bin(self._h, path.encode(), sim_id.encode(), int(step)))
+
+    def save_info(self, path: str, sim_id: str, total_steps: int, mlups: float, seconds: float) -> None:
+        check(lib.vlbm_save_info(self._h, path.encode(), sim_id.encode(), int(total_steps),
+                                 float(mlups), float(seconds)))
