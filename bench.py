@@ -84,12 +84,10 @@ def main():
               "multi-GPU path in this build", file=sys.stderr)
         sys.exit(2)
 
+    # torch is not imported: its wheel bundles its own HIP runtime, and two HIP
+    # runtimes in one process do not share the device. All work runs on the
+    # library's stream, so vlbm_sync() brackets the timed region.
     import vlbm
-    try:
-        import torch
-        sync_torch = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only
-        sync_torch = lambda: None
 
     cfg = workload_config("single")
     init = initial_fields()
@@ -97,12 +95,10 @@ def main():
     lat.init(*init)
     lat.step(args.warmup)
     lat.sync()
-    sync_torch()
 
     t0 = time.perf_counter()
     event_ms = lat.time_steps(args.steps)      # HIP events on the library stream around the K steps
     lat.sync()
-    sync_torch()
     wall = time.perf_counter() - t0
 
     nodes = N_SIDE ** 3

@@ -44,18 +44,33 @@ def pops_dev(p, q):
     return p[:, :-1] - weights(q)[:, None, None, None]
 
 
+def used_mask(lat):
+    """Nodes that collide; populations at unused nodes (outside a duct wall)
+    are never read by the step and hold unspecified values on both sides."""
+    words = vlbm.build_map(lat.desc)
+    return ((words >> 21) & 1).astype(bool)
+
+
 def assert_fp64_parity(lat, orc, q, tol=1e-10, omega=False):
-    gp, op = lat.get_pops(), orc.get_pops()
-    assert rel_l2(pops_dev(gp, q), pops_dev(op, q)) <= tol
+    used = used_mask(lat)
+    gp, op = pops_dev(lat.get_pops(), q), pops_dev(orc.get_pops(), q)
+    err = rel_l2(gp[:, used], op[:, used])
+    if err > tol:
+        diff = np.abs(gp - op) * used[None]
+        i, z, y, x = np.unravel_index(np.argmax(diff), diff.shape)
+        print(f"pops rel-L2 {err:.3e}; worst pop {i} at (x,y,z)=({x},{y},{z}): "
+              f"gpu {gp[i, z, y, x]:.6e} oracle {op[i, z, y, x]:.6e}", flush=True)
+    assert err <= tol
     gm, om = lat.macroscopics(omega=omega), orc.macroscopics()
-    for k in ("ux", "uy", "uz"):
-        if np.linalg.norm(om[k]) > 0:
-            assert rel_l2(gm[k], om[k]) <= tol, k
-        else:
-            assert np.max(np.abs(gm[k])) <= 1e-14
+    u_g = np.stack([gm["ux"], gm["uy"], gm["uz"]])
+    u_o = np.stack([om["ux"], om["uy"], om["uz"]])
+    if np.linalg.norm(u_o) > 0:
+        assert rel_l2(u_g, u_o) <= tol
+    else:
+        assert np.max(np.abs(u_g)) <= 1e-14
     assert rel_l2(gm["rho"], om["rho"]) <= tol
     if omega:
-        assert rel_l2(gm["omega"], om["omega"]) <= tol
+        assert rel_l2(gm["omega"][used], om["omega"][used]) <= tol
 
 
 # ---------------------------------------------------------------------------

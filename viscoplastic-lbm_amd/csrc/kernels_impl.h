@@ -279,7 +279,10 @@ __device__ void interp_bounce_back(T* g, const Params<T>& p, const Node<T>& nd, 
                 const T fi = load_pop<S, T>(p, i, self);
                 g[i] = ((T)0.5 / qd) * fj + ((qd - (T)0.5) / qd) * fi + (T)3 * df / qd;
             } else {
-                const int xa = nd.x + kCX[i], ya = nd.y + kCY[i];
+                // the reference reads x + c_i unwrapped; wrapping keeps degenerate
+                // (1-2 node wide) lattices in bounds and is the same node otherwise
+                const int xa = kCX[i] > 0 ? nd.xp : (kCX[i] < 0 ? nd.xm : nd.x);
+                const int ya = kCY[i] > 0 ? nd.yp : (kCY[i] < 0 ? nd.ym : nd.y);
                 const int za = kCZ[i] > 0 ? nd.zp : (kCZ[i] < 0 ? nd.zm : nd.z);
                 const T fa = load_pop<S, T>(p, j, row_off(p, ya, za) + (uint32_t)xa);
                 g[i] = 2 * qd * fj + (1 - 2 * qd) * fa + (T)6 * df;
