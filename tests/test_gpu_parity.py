@@ -240,3 +240,42 @@ def test_c2_fp32_vs_fp64_full_size():
     u32 = np.stack([out["single"][k] for k in ("ux", "uy", "uz")])
     u64 = np.stack([out["double"][k] for k in ("ux", "uy", "uz")])
     assert rel_l2(u32, u64) <= 1e-4
+
+
+# ---------------------------------------------------------------------------
+# driver loop (main.cu cadence) and output files
+# ---------------------------------------------------------------------------
+def test_runner_cadence_and_final_file(tmp_path):
+    """Saves at multiples of macr_save labelled with the step, the final file
+    labelled N holds the moments computed during step N-1 (main.cu:296,524-527)."""
+    announce("runner cadence")
+    from vlbm.runner import run
+    cfg = base_cfg(precision="double", NX=4, NY=16, NZ=4, tau=0.8, FX=1e-5, builder="parallelPlatesBounceBack_Y",
+                   steps=250, macr_save=100, data_report=50, path_save=str(tmp_path), sim_id=7)
+    res = run(cfg, log=lambda s: None)
+    assert res.steps == 250 and [r["step"] for r in res.reports] == [50, 100, 150, 200]
+    names = sorted(p.name for p in (tmp_path / "007").iterdir())
+    for lab in ("000100", "000200", "000250"):
+        assert f"007_ux{lab}.bin" in names
+    assert "007_info.txt" in names
+    orc = po.Oracle(po.desc_from(cfg.to_desc()), 8)
+    orc.init()
+    orc.step(100, save_last=True)     # the step labelled 99 -> state after 99 steps; label 100 needs 101
+    orc.step(1, save_last=True)
+    ux100 = np.fromfile(tmp_path / "007" / "007_ux000100.bin", dtype="d").reshape(4, 16, 4)
+    assert rel_l2(ux100, orc.macroscopics()["ux"]) < 1e-12
+    orc.step(149, save_last=True)     # 250 steps executed: moments of the state after 249 steps
+    ux_final = np.fromfile(tmp_path / "007" / "007_ux000250.bin", dtype="d").reshape(4, 16, 4)
+    assert rel_l2(ux_final, orc.macroscopics()["ux"]) < 1e-12
+
+
+def test_runner_residual_stop(tmp_path):
+    announce("runner residual stop")
+    from vlbm.runner import run
+    cfg = base_cfg(precision="double", NX=2, NY=12, NZ=2, tau=1.2, FX=1e-6, builder="parallelPlatesBounceBack_Y",
+                   steps=200000, macr_save=0, data_report=500, data_stop=True, resid_max=1e-6,
+                   path_save=str(tmp_path), sim_id=3)
+    res = run(cfg, log=lambda s: None)
+    assert res.stopped == "residual"
+    assert res.reports[-1]["residual"] < 1e-6 <= res.reports[-2]["residual"]
+    assert res.steps == res.reports[-1]["step"]
