@@ -390,6 +390,7 @@ struct Lattice {
     std::vector<uint32_t> map;
     std::vector<size_t> bc_list;
     std::vector<R> rho, ux, uy, uz, omega;
+    std::vector<R> force_x, force_y, force_z;   // per-node force (IBM builds only)
     R w[27];
 
     size_t ip(int x, int y, int z, int i) const {
@@ -1052,7 +1053,14 @@ struct Lattice {
                     const uint32_t m = map[k];
                     if (!w_used(m)) continue;
                     for (int i = 0; i < q; i++) f[i] = pop[ip(x, y, z, i)];
-                    collide(f, FXv, FYv, FZv, &omega[k], mo);
+                    if (!force_x.empty()) {
+                        // IBM builds read the per-node force and reset it to the
+                        // constant force (src/CUDA/lbm.cu:45-58)
+                        collide(f, force_x[k], force_y[k], force_z[k], &omega[k], mo);
+                        force_x[k] = FXv; force_y[k] = FYv; force_z[k] = FZv;
+                    } else {
+                        collide(f, FXv, FYv, FZv, &omega[k], mo);
+                    }
                     if (save) {
                         rho[k] = mo[0]; ux[k] = mo[1]; uy[k] = mo[2]; uz[k] = mo[3];
                         if (d.rheo == 1) omega[k] = mo[4];
@@ -1081,9 +1089,566 @@ struct Lattice {
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// Immersed boundary stage for spheres (src/CUDA/IBM/ibm.cu:6-1197,
+// collision/ibmCollision.cu), FP64, serial loops in the reference's order.
+// ---------------------------------------------------------------------------
+} // namespace
+
+extern "C" {
+struct or_ibm_desc {
+    int max_iterations;                 // IBM_MAX_ITERATION
+    int periodic_x, periodic_y, periodic_z;   // IBM_BC_{X,Y,Z}_PERIODIC (else wall)
+    double thickness;                   // IBM_THICKNESS
+    double alpha;                       // IBM_MOVEMENT_DISCRETIZATION
+    double fluid_density;               // FLUID_DENSITY
+    double gx, gy, gz;                  // GX, GY, GZ
+    double pp_friction, pw_friction, pp_rest, pw_rest;
+    double young, poisson, wall_young, wall_poisson;
+    int ref_float_helpers;              // 1: float sqrt/rsqrt helpers as the reference
+};
+}
+
+namespace {
+
+constexpr int kMaxColl = 13;
+
+struct PCenter {
+    double pos[3], pos_old[3], vel[3], vel_old[3], w[3], w_avg[3], w_old[3], w_pos[3];
+    double f[3], f_old[3], M[3], M_old[3];
+    double I[6];   // xx yy zz xy xz yz
+    double dP[3], dL[3];
+    double S, radius, volume, density;
+    bool movable;
+    int partner[kMaxColl];
+    double tang[kMaxColl][3];
+    int last[kMaxColl];
+};
+
+// Peskin 4-point kernel (src/CUDA/IBM/ibmGlobalFunctions.h:28-48)
+inline double stencil4(double x) {
+    const double a = std::fabs(x);
+    if (a <= 1) return (1.0 / 8.0) * (3.0 - 2.0 * a + std::sqrt(1 + 4 * a - 4 * a * a));
+    if (a > 1.0 && a <= 2.0) return (1.0 / 8.0) * (5.0 - 2.0 * a - std::sqrt(-7.0 + 12.0 * a - 4.0 * a * a));
+    return 0.0;
+}
+
+struct Ibm {
+    Lattice<double>* L;
+    or_ibm_desc d;
+    std::vector<PCenter> pc;
+    std::vector<double> px, py, pz, nS, nfx, nfy, nfz, ndx, ndy, ndz;
+    std::vector<int> nc;
+    std::vector<double> velAux[3], fAux[3];
+
+    double pp_kn, pw_kn, pp_kt, pw_kt;
+
+    Ibm(Lattice<double>* lat, const or_ibm_desc& desc) : L(lat), d(desc) {
+        const size_t n = L->nnodes;
+        for (int a = 0; a < 3; a++) { velAux[a].assign(n, 0.0); fAux[a].assign(n, 0.0); }
+        L->force_x.assign(n, L->d.fx); L->force_y.assign(n, L->d.fy); L->force_z.assign(n, L->d.fz);
+        // src/CUDA/IBM/ibmBoundaryCondition.h:77-96
+        const double pG = d.young / (2.0 + 2.0 * d.poisson), wG = d.wall_young / (2.0 + 2.0 * d.wall_poisson);
+        const double S2 = 1.41421356237309504880168872420969807856967187537;
+        pp_kn = (4.0 / 3.0) / ((1 - d.poisson * d.poisson) / d.young + (1 - d.poisson * d.poisson) / d.young);
+        pw_kn = (4.0 / 3.0) / ((1 - d.poisson * d.poisson) / d.young + (1 - d.wall_poisson * d.wall_poisson) / d.wall_young);
+        pp_kt = 4.0 * S2 / ((2 - d.poisson) / pG + (2 - d.poisson) / pG);
+        pw_kt = 4.0 * S2 / ((2 - d.poisson) / pG + (2 - d.wall_poisson) / wG);
+    }
+
+    double vlen(const double v[3]) const {
+        if (d.ref_float_helpers) return (double)sqrtf((float)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+        return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    void vnorm(const double v[3], double o[3]) const {
+        double inv;
+        if (d.ref_float_helpers) inv = (double)(1.0f / sqrtf((float)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])));
+        else inv = 1.0 / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (std::isnan(inv) || std::isinf(inv)) { o[0] = o[1] = o[2] = 0; return; }
+        o[0] = v[0] * inv; o[1] = v[1] * inv; o[2] = v[2] * inv;
+    }
+
+    int axis_n(int a) const { return a == 0 ? L->nx : (a == 1 ? L->ny : L->nz); }
+    bool periodic(int a) const { return a == 0 ? d.periodic_x : (a == 1 ? d.periodic_y : d.periodic_z); }
+
+    // gpuUpdateParticleOldValues (ibm.cu:995-1042); M_old is not updated there
+    void old_values() {
+        for (auto& c : pc) {
+            for (int a = 0; a < 3; a++) {
+                c.dP[a] = 0; c.dL[a] = 0;
+                c.pos_old[a] = c.pos[a]; c.vel_old[a] = c.vel[a]; c.w_old[a] = c.w[a];
+                c.f_old[a] = c.f[a]; c.f[a] = 0; c.M[a] = 0;
+            }
+        }
+    }
+
+    // gpuUpdateMacrIBM (ibm.cu:544-648): rho, u of the post-step populations
+    // with the constant force; auxiliaries zeroed.
+    void update_macr() {
+        const int q = L->q;
+        const double F[3] = {L->d.fx, L->d.fy, L->d.fz};
+        for (int z = 0; z < L->nz; z++)
+            for (int y = 0; y < L->ny; y++)
+                for (int x = 0; x < L->nx; x++) {
+                    const size_t k = L->is(x, y, z);
+                    for (int a = 0; a < 3; a++) { velAux[a][k] = 0; fAux[a][k] = 0; }
+                    double f[27] = {};
+                    for (int i = 0; i < q; i++) f[i] = L->pop[L->ip(x, y, z, i)];
+                    double r = f[0];
+                    for (int i = 1; i < q; i++) r += f[i];
+                    const double inv = 1 / r;
+                    double j[3] = {0, 0, 0};
+                    double jp[3] = {0, 0, 0}, jm[3] = {0, 0, 0};
+                    for (int i = 1; i < q; i++) {
+                        const int c[3] = {CX[i], CY[i], CZ[i]};
+                        for (int a = 0; a < 3; a++) { if (c[a] > 0) jp[a] += f[i]; else if (c[a] < 0) jm[a] += f[i]; }
+                    }
+                    for (int a = 0; a < 3; a++) j[a] = (jp[a] - jm[a] + 0.5 * F[a]) * inv;
+                    L->rho[k] = r; L->ux[k] = j[0]; L->uy[k] = j[1]; L->uz[k] = j[2];
+                }
+    }
+
+    // ---- contacts (collision/ibmCollision.cu) ----
+    static int wall_index(const double n[3]) {
+        return 7 + (1 - (int)n[0]) - 2 * (1 - (int)n[1]) - 3 * (1 - (int)n[2]);
+    }
+    void end_collision(PCenter& c, int idx, int step) {
+        if (idx >= 0 && idx <= 6) c.last[idx] = -1;
+        else if (idx >= 7 && idx < kMaxColl) {
+            if (step - c.last[idx] > 1) {
+                c.tang[idx][0] = c.tang[idx][1] = c.tang[idx][2] = 0;
+                c.last[idx] = -1; c.partner[idx] = -1;
+            }
+        }
+    }
+    int start_collision(PCenter& c, int partner, bool wall, const double n[3], int step) {
+        int index = -1;
+        if (wall) {
+            index = wall_index(n);
+            c.tang[index][0] = c.tang[index][1] = c.tang[index][2] = 0;
+            c.last[index] = step;
+        } else {
+            for (int i = 7; i < kMaxColl; i++)
+                if (c.last[i] == -1) {
+                    c.partner[i] = partner; c.tang[i][0] = c.tang[i][1] = c.tang[i][2] = 0;
+                    c.last[i] = step; index = i; break;
+                }
+        }
+        return index;
+    }
+    void update_tang(PCenter& c, int idx, const double dsp[3], int step, double out[3]) {
+        for (int a = 0; a < 3; a++) { c.tang[idx][a] += dsp[a]; out[a] = c.tang[idx][a]; }
+        c.last[idx] = step;
+    }
+    int index_by_partner(const PCenter& c, int partner, int step) const {
+        for (int i = 7; i < kMaxColl; i++)
+            if (c.partner[i] == partner && step - c.last[i] <= 1) return i;
+        return -1;
+    }
+
+    // sphereWallCollision (ibmCollision.cu:394-508)
+    void sphere_wall(PCenter& c, const double wall_normal[3], double displacement, int step) {
+        const double m_i = c.volume * c.density, r_i = c.radius;
+        double n[3] = {-wall_normal[0], -wall_normal[1], -wall_normal[2]};
+        const double* G = c.vel;
+        const double* w_i = c.w;
+        const double kn = pw_kn * std::sqrt(std::fabs(r_i));
+        const double kt = pw_kt * std::sqrt(r_i) * std::sqrt(std::fabs(displacement));
+        const double lr = std::log(d.pw_rest);
+        const double dc = (-2.0 * lr / (std::sqrt(M_PI * M_PI + lr * lr)));
+        const double dn = dc * std::sqrt(m_i * kn), dt = dc * std::sqrt(m_i * kt);
+        const double f_kn = -kn * std::sqrt(std::fabs(displacement * displacement * displacement));
+        const double Gn = G[0] * n[0] + G[1] * n[1] + G[2] * n[2];
+        const double p25 = std::pow(std::fabs(displacement), 0.25);
+        double fn[3];
+        for (int a = 0; a < 3; a++) fn[a] = f_kn * n[a] - dn * Gn * n[a] * p25;
+        const double f_n = std::sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+        double Gct[3];
+        Gct[0] = G[0] + r_i * (w_i[1] * n[2] - w_i[2] * n[1]) - Gn * n[0];
+        Gct[1] = G[1] + r_i * (w_i[2] * n[0] - w_i[0] * n[2]) - Gn * n[1];
+        Gct[2] = G[2] + r_i * (w_i[0] * n[1] - w_i[1] * n[0]) - Gn * n[2];
+        double mag = std::sqrt(Gct[0] * Gct[0] + Gct[1] * Gct[1] + Gct[2] * Gct[2]);
+        double t[3] = {0, 0, 0};
+        if (mag != 0) for (int a = 0; a < 3; a++) t[a] = Gct[a] / mag;
+        const int ti = wall_index(n);
+        double td[3];
+        if (step - c.last[ti] > 1) {
+            end_collision(c, ti, step);
+            start_collision(c, ti, true, n, step);
+            for (int a = 0; a < 3; a++) td[a] = Gct[a];
+        } else {
+            update_tang(c, ti, Gct, step, td);
+        }
+        double ft[3];
+        for (int a = 0; a < 3; a++) ft[a] = -kt * td[a] - dt * Gct[a] * std::pow(std::fabs(td[a]), 0.25);
+        mag = std::sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+        const double fnab = d.ref_float_helpers ? (double)fabsf((float)f_n) : std::fabs(f_n);
+        if (mag > d.pw_friction * fnab) {
+            const double mg[3] = {-Gct[0], -Gct[1], -Gct[2]};
+            update_tang(c, ti, mg, step, td);
+            for (int a = 0; a < 3; a++) ft[a] = -d.pw_friction * f_n * t[a];
+        }
+        const double fd[3] = {fn[0] + ft[0], fn[1] + ft[1], fn[2] + ft[2]};
+        const double md[3] = {r_i * (n[1] * ft[2] - n[2] * ft[1]), r_i * (n[2] * ft[0] - n[0] * ft[2]),
+                              r_i * (n[0] * ft[1] - n[1] * ft[0])};
+        for (int a = 0; a < 3; a++) { c.f[a] += fd[a]; c.M[a] += md[a]; }
+    }
+
+    // checkCollisionWallsSphere (ibmCollision.cu:188-251): walls at 0 and N-1
+    void walls(PCenter& c, int step) {
+        for (int a = 0; a < 3; a++) {
+            if (periodic(a)) continue;
+            double nrm[3] = {0, 0, 0};
+            nrm[a] = 1;
+            double dist = c.pos[a] * 1.0 - 0.0;
+            if (dist < c.radius) sphere_wall(c, nrm, c.radius - dist, step);
+            nrm[a] = -1;
+            dist = (double)(axis_n(a) - 1) + (-c.pos[a]);
+            if (dist < c.radius) sphere_wall(c, nrm, c.radius - dist, step);
+        }
+    }
+
+    double min_image(int a, double di, double pi, double pj) const {
+        if (!periodic(a)) return di;
+        const double Lx = axis_n(a);
+        if (std::fabs(pi - pj) > Lx / 2.0) return pi < pj ? (pi + Lx - pj) : (pi - Lx - pj);
+        return di;
+    }
+
+    // sphereSphereGap (ibmCollision.cu:1061-1088): wrap with N/2, float sqrt in the reference
+    double gap(const PCenter& a, const PCenter& b) const {
+        double dl[3];
+        for (int k = 0; k < 3; k++) {
+            dl[k] = a.pos[k] - b.pos[k];
+            if (periodic(k)) {
+                const double Lk = axis_n(k);
+                if (dl[k] > Lk / 2.0) dl[k] -= Lk;
+                if (dl[k] < -Lk / 2.0) dl[k] += Lk;
+            }
+        }
+        const double s2 = dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2];
+        const double dist = d.ref_float_helpers ? (double)sqrtf((float)s2) : std::sqrt(s2);
+        return dist - (a.radius + b.radius);
+    }
+
+    // sphereSphereCollision (ibmCollision.cu:1719-1913)
+    void sphere_sphere(unsigned row, PCenter& ci, PCenter& cj, int step) {
+        const double r_i = ci.radius, r_j = cj.radius;
+        const double m_i = ci.volume * ci.density, m_j = cj.volume * cj.density;
+        double dp[3];
+        for (int a = 0; a < 3; a++) dp[a] = min_image(a, ci.pos[a] - cj.pos[a], ci.pos[a], cj.pos[a]);
+        const double md = std::sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+        if (md > r_i + r_j) return;
+        const double n[3] = {dp[0] / md, dp[1] / md, dp[2] / md};
+        const double displacement = r_i + r_j - md;
+        const double G[3] = {ci.vel[0] - cj.vel[0], ci.vel[1] - cj.vel[1], ci.vel[2] - cj.vel[2]};
+        const double er = 1.0 / ((r_i + r_j) / (r_i * r_j));
+        const double em = 1.0 / ((m_i + m_j) / (m_i * m_j));
+        const double kn = pp_kn * std::sqrt(er);
+        const double kt = pp_kt * std::sqrt(er) * std::sqrt(std::fabs(displacement));
+        const double lr = std::log(d.pp_rest);
+        const double dc = (-2.0 * lr / (std::sqrt(M_PI * M_PI + lr * lr)));
+        const double dn = dc * std::sqrt(em * kn), dt = dc * std::sqrt(em * kt);
+        const double f_kn = -kn * std::sqrt(std::fabs(displacement * displacement * displacement));
+        const double Gn = G[0] * n[0] + G[1] * n[1] + G[2] * n[2];
+        const double p25 = std::pow(std::fabs(displacement), 0.25);
+        double fn[3];
+        for (int a = 0; a < 3; a++) fn[a] = f_kn * n[a] - dn * Gn * n[a] * p25;
+        const double f_n = std::sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+        const double* wi = ci.w;
+        const double* wj = cj.w;
+        double Gct[3];
+        Gct[0] = G[0] + r_i * (wi[1] * n[2] - wi[2] * n[1]) + r_j * (wj[1] * n[2] - wj[2] * n[1]) - Gn * n[0];
+        Gct[1] = G[1] + r_i * (wi[2] * n[0] - wi[0] * n[2]) + r_j * (wj[2] * n[0] - wj[0] * n[2]) - Gn * n[1];
+        Gct[2] = G[2] + r_i * (wi[0] * n[1] - wi[1] * n[0]) + r_j * (wj[0] * n[1] - wj[1] * n[0]) - Gn * n[2];
+        double mag = std::sqrt(Gct[0] * Gct[0] + Gct[1] * Gct[1] + Gct[2] * Gct[2]);
+        double t[3] = {0, 0, 0};
+        if (mag != 0) for (int a = 0; a < 3; a++) t[a] = Gct[a] / mag;
+        double td[3];
+        int ti = index_by_partner(ci, (int)row, step);
+        if (ti == -1) {
+            ti = start_collision(ci, (int)row, false, n, step);
+            for (int a = 0; a < 3; a++) td[a] = Gct[a];
+        } else if (step - ci.last[ti] > 1) {
+            end_collision(ci, ti, step);
+            ti = start_collision(ci, (int)row, false, n, step);
+            for (int a = 0; a < 3; a++) td[a] = Gct[a];
+        } else {
+            update_tang(ci, ti, G, step, td);   // the reference accumulates G here, not G_ct
+        }
+        double ft[3];
+        for (int a = 0; a < 3; a++) ft[a] = -kt * td[a] - dt * Gct[a] * std::pow(std::fabs(td[a]), 0.25);
+        mag = std::sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+        if (mag > d.pp_friction * std::fabs(f_n)) {
+            const double mg[3] = {-Gct[0], -Gct[1], -Gct[2]};
+            if (ti >= 0) update_tang(ci, ti, mg, step, td);
+            for (int a = 0; a < 3; a++) ft[a] = -d.pp_friction * f_n * t[a];
+        }
+        const double fd[3] = {fn[0] + ft[0], fn[1] + ft[1], fn[2] + ft[2]};
+        const double mi[3] = {r_i * (n[1] * ft[2] - n[2] * ft[1]), r_i * (n[2] * ft[0] - n[0] * ft[2]),
+                              r_i * (n[0] * ft[1] - n[1] * ft[0])};
+        const double mj[3] = {r_j * (n[1] * ft[2] - n[2] * ft[1]), r_j * (n[2] * ft[0] - n[0] * ft[2]),
+                              r_j * (n[0] * ft[1] - n[1] * ft[0])};
+        for (int a = 0; a < 3; a++) {
+            ci.f[a] += -fd[a]; ci.M[a] += mi[a];
+            cj.f[a] += fd[a]; cj.M[a] += mj[a];
+        }
+    }
+
+    // gpuParticlesCollisionHandler (ibmCollision.cu:2381-2448), Floyd-triangle order;
+    // the reference's guard admits idx == total, which reads past the array: skipped.
+    void collisions(int step) {
+        const unsigned np = (unsigned)pc.size();
+        const unsigned total = np * (np + 1) / 2;
+        for (unsigned idx = 0; idx < total; idx++) {
+            const unsigned row = (unsigned)std::ceil((-1.0 + std::sqrt((double)1 + 8 * (idx + 1))) / 2);
+            const unsigned column = idx - ((row - 1) * row) / 2;
+            PCenter& ci = pc[column];
+            if (row == np) {
+                if (!ci.movable) continue;
+                walls(ci, step);
+            } else {
+                PCenter& cj = pc[row];
+                if (!ci.movable && !cj.movable) continue;
+                if (gap(ci, cj) < 0) sphere_sphere(row, ci, cj, step);
+            }
+        }
+    }
+
+    // gpuUpdateParticleCenterVelocityAndRotation (ibm.cu:755-897)
+    void update_velocity() {
+        const double al = d.alpha;
+        const double g[3] = {d.gx, d.gy, d.gz};
+        for (auto& c : pc) {
+            if (!c.movable) continue;
+            const double inv_volume = 1 / c.volume;
+            for (int a = 0; a < 3; a++)
+                c.vel[a] = c.vel_old[a] + (((c.f_old[a] * (1.0 - al) + c.f[a] * al) + c.dP[a]) * inv_volume +
+                                           (c.density - d.fluid_density) * g[a]) / (c.density);
+            const double* I = c.I;
+            const double xx = I[0], yy = I[1], zz = I[2], xy = I[3], xz = I[4], yz = I[5];
+            double wAux[3] = {c.w_old[0], c.w_old[1], c.w_old[2]};
+            const double inv_det_neg = 1.0 / (zz * xy * xy + yy * xz * xz + xx * yz * yz - xx * yy * zz - 2 * xy * xz * yz);
+            double wA[3], LM[3];
+            for (int a = 0; a < 3; a++) {
+                wA[a] = (c.w_old[a] * (1.0 - al) + wAux[a] * al);
+                LM[a] = c.dL[a] + (c.M[a] * al + c.M_old[a] * (1.0 - al));
+            }
+            // I w components
+            const double Iwx = xx * wA[0] + xy * wA[1] + xz * wA[2];
+            const double Iwy = xy * wA[0] + yy * wA[1] + yz * wA[2];
+            const double Iwz = xz * wA[0] + yz * wA[1] + zz * wA[2];
+            const double Ax = LM[0] + wA[2] * Iwy - wA[1] * Iwz;
+            const double Ay = LM[1] + wA[0] * Iwz - wA[2] * Iwx;
+            const double Az = LM[2] + wA[1] * Iwx - wA[0] * Iwy;
+            double wN[3];
+            wN[0] = c.w_old[0] + ((yz * yz - yy * zz) * Ax - (xy * yz - xz * yy) * Az - (xz * yz - xy * zz) * Ay) * inv_det_neg;
+            wN[1] = c.w_old[1] + ((xz * xz - xx * zz) * Ay - (xy * xz - xx * yz) * Az - (xz * yz - xy * zz) * Ax) * inv_det_neg;
+            wN[2] = c.w_old[2] + ((xy * xy - xx * yy) * Az - (xy * xz - xx * yz) * Ay - (xy * yz - xz * yy) * Ax) * inv_det_neg;
+            // inertia rotated by the average rotation (axis_angle_to_quart / rotate_inertia_by_quart)
+            const double ang = vlen(wA);
+            double ax[3];
+            vnorm(wA, ax);
+            const double qw = std::cos(ang / 2.0), qx = ax[0] * std::sin(ang / 2.0), qy = ax[1] * std::sin(ang / 2.0),
+                         qz = ax[2] * std::sin(ang / 2.0);
+            double R[3][3];
+            R[0][0] = 1 - 2 * (qy * qy + qz * qz); R[0][1] = 2 * (qx * qy - qw * qz); R[0][2] = 2 * (qx * qz + qw * qy);
+            R[1][0] = 2 * (qx * qy + qw * qz); R[1][1] = 1 - 2 * (qx * qx + qz * qz); R[1][2] = 2 * (qy * qz - qw * qx);
+            R[2][0] = 2 * (qx * qz - qw * qy); R[2][1] = 2 * (qy * qz + qw * qx); R[2][2] = 1 - 2 * (qx * qx + qy * qy);
+            const double M3[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
+            double T[3][3], N3[3][3];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    T[i][j] = 0;
+                    for (int k = 0; k < 3; k++) T[i][j] += R[i][k] * M3[k][j];
+                }
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    N3[i][j] = 0;
+                    for (int k = 0; k < 3; k++) N3[i][j] += T[i][k] * R[j][k];
+                }
+            for (int a = 0; a < 3; a++) c.w[a] = wN[a];
+            c.I[0] = N3[0][0]; c.I[1] = N3[1][1]; c.I[2] = N3[2][2];
+            c.I[3] = N3[0][1]; c.I[4] = N3[0][2]; c.I[5] = N3[1][2];
+        }
+    }
+
+    // gpuForceInterpolationSpread (ibm.cu:194-542), walls clip the stencil,
+    // periodic axes wrap it (single GPU).
+    void interp_spread() {
+        const size_t nn = px.size();
+        for (size_t i = 0; i < nn; i++) {
+            const double pos[3] = {px[i], py[i], pz[i]};
+            int base[3], mn[3], mx[3];
+            for (int a = 0; a < 3; a++) {
+                base[a] = (int)pos[a] - 2 + 1;
+                const int Na = axis_n(a);
+                if (periodic(a)) { mn[a] = 0; mx[a] = 3; }
+                else {
+                    mx[a] = ((base[a] + 3) < Na) ? 3 : (Na - 1 - base[a]);
+                    mn[a] = (base[a] >= 0) ? 0 : -base[a];
+                }
+            }
+            if (mx[0] <= 0 || mx[1] <= 0 || mx[2] <= 0) continue;
+            if (mn[0] >= 4 || mn[1] >= 4 || mn[2] >= 4) continue;
+            double sv[3][4];
+            for (int a = 0; a < 3; a++)
+                for (int j = mn[a]; j <= mx[a]; j++) sv[a][j] = stencil4(base[a] + j - pos[a]);
+            auto node_idx = [&](int xi, int yj, int zk) -> size_t {
+                int c3[3] = {base[0] + xi, base[1] + yj, base[2] + zk};
+                for (int a = 0; a < 3; a++)
+                    if (periodic(a)) { const int Na = axis_n(a); c3[a] = (c3[a] + Na) % Na; }
+                return L->is(c3[0], c3[1], c3[2]);
+            };
+            double rv = 0, uxv = 0, uyv = 0, uzv = 0;
+            for (int zk = mn[2]; zk <= mx[2]; zk++)
+                for (int yj = mn[1]; yj <= mx[1]; yj++) {
+                    const double a1 = sv[2][zk] * sv[1][yj];
+                    for (int xi = mn[0]; xi <= mx[0]; xi++) {
+                        const double a = a1 * sv[0][xi];
+                        const size_t k = node_idx(xi, yj, zk);
+                        rv += L->rho[k] * a; uxv += L->ux[k] * a; uyv += L->uy[k] * a; uzv += L->uz[k] * a;
+                    }
+                }
+            PCenter& c = pc[nc[i]];
+            double dx = px[i] - c.pos[0], dy = py[i] - c.pos[1], dz = pz[i] - c.pos[2];
+            double* dd[3] = {&dx, &dy, &dz};
+            for (int a = 0; a < 3; a++) {
+                if (!periodic(a)) continue;
+                const double Na = axis_n(a);
+                if (std::fabs(*dd[a]) > Na / 2.0) *dd[a] = (*dd[a] < 0) ? (pos[a] + Na) - c.pos[a] : (pos[a] - Na) - c.pos[a];
+            }
+            double uc[3] = {0, 0, 0};
+            if (c.movable) {
+                uc[0] = c.vel[0] + (c.w[1] * dz - c.w[2] * dy);
+                uc[1] = c.vel[1] + (c.w[2] * dx - c.w[0] * dz);
+                uc[2] = c.vel[2] + (c.w[0] * dy - c.w[1] * dx);
+            }
+            const double aux = 2 * rv * nS[i] * d.thickness;
+            const double dF[3] = {aux * (uxv - uc[0]), aux * (uyv - uc[1]), aux * (uzv - uc[2])};
+            for (int zk = mn[2]; zk <= mx[2]; zk++)
+                for (int yj = mn[1]; yj <= mx[1]; yj++) {
+                    const double a1 = sv[2][zk] * sv[1][yj];
+                    for (int xi = mn[0]; xi <= mx[0]; xi++) {
+                        const double a = a1 * sv[0][xi];
+                        const size_t k = node_idx(xi, yj, zk);
+                        fAux[0][k] += -dF[0] * a; fAux[1][k] += -dF[1] * a; fAux[2][k] += -dF[2] * a;
+                        const double ir = 1 / L->rho[k];
+                        velAux[0][k] += 0.5 * -dF[0] * a * ir;
+                        velAux[1][k] += 0.5 * -dF[1] * a * ir;
+                        velAux[2][k] += 0.5 * -dF[2] * a * ir;
+                    }
+                }
+            nfx[i] += dF[0]; nfy[i] += dF[1]; nfz[i] += dF[2];
+            ndx[i] = dF[0]; ndy[i] = dF[1]; ndz[i] = dF[2];
+            const double dm[3] = {dy * dF[2] - dz * dF[1], dz * dF[0] - dx * dF[2], dx * dF[1] - dy * dF[0]};
+            for (int a = 0; a < 3; a++) { c.f[a] += dF[a]; c.M[a] += dm[a]; }
+        }
+    }
+
+    // gpuEulerSumIBMAuxsReset (particleEulerNodesUpdate.cu:287-330)
+    void euler_sum() {
+        for (size_t k = 0; k < L->nnodes; k++) {
+            L->ux[k] += velAux[0][k]; L->uy[k] += velAux[1][k]; L->uz[k] += velAux[2][k];
+            L->force_x[k] += fAux[0][k]; L->force_y[k] += fAux[1][k]; L->force_z[k] += fAux[2][k];
+            for (int a = 0; a < 3; a++) { velAux[a][k] = 0; fAux[a][k] = 0; }
+        }
+    }
+
+    double wrap_pos(int a, double v) const {
+        const double Na = axis_n(a);
+        return std::fmod(v + Na, Na);
+    }
+
+    // gpuParticleMovement (ibm.cu:899-993); semi-axes are not used by spheres
+    void movement() {
+        const double al = d.alpha;
+        for (auto& c : pc) {
+            if (!c.movable) continue;
+            for (int a = 0; a < 3; a++) {
+                const double dp = (c.vel[a] * al + c.vel_old[a] * (1.0 - al));
+                if (periodic(a)) c.pos[a] = wrap_pos(a, c.pos[a] + dp);
+                else c.pos[a] += dp;
+            }
+            for (int a = 0; a < 3; a++) {
+                c.w_avg[a] = (c.w[a] * al + c.w_old[a] * (1.0 - al));
+                c.w_pos[a] += c.w_avg[a];
+            }
+        }
+    }
+
+    // gpuParticleNodeMovement (ibm.cu:1044-1197), including the reference's
+    // (qi*qj - q0*qj) entry of the rotation.
+    void node_movement() {
+        for (size_t i = 0; i < px.size(); i++) {
+            const PCenter& c = pc[nc[i]];
+            if (!c.movable) continue;
+            double* p3[3] = {&px[i], &py[i], &pz[i]};
+            const double wn = std::sqrt(c.w_avg[0] * c.w_avg[0] + c.w_avg[1] * c.w_avg[1] + c.w_avg[2] * c.w_avg[2]);
+            if (wn <= 1e-8) {
+                for (int a = 0; a < 3; a++) {
+                    double dd = c.pos[a] - c.pos_old[a];
+                    if (periodic(a)) {
+                        const double Na = axis_n(a);
+                        if (std::fabs(dd) > Na / 2.0) dd = (c.pos[a] < c.pos_old[a]) ? (c.pos[a] + Na) - c.pos_old[a] : (c.pos[a] - Na) - c.pos_old[a];
+                        *p3[a] = wrap_pos(a, *p3[a] + dd);
+                    } else {
+                        *p3[a] += dd;
+                    }
+                }
+                continue;
+            }
+            double v[3];
+            for (int a = 0; a < 3; a++) {
+                v[a] = *p3[a] - c.pos_old[a];
+                if (periodic(a)) {
+                    const double Na = axis_n(a);
+                    if (std::fabs(v[a]) > Na / 2.0) {
+                        if (*p3[a] < c.pos_old[a]) *p3[a] += Na; else *p3[a] -= Na;
+                    }
+                    v[a] = *p3[a] - c.pos_old[a];
+                }
+            }
+            const double q0 = std::cos(0.5 * wn);
+            const double qi = (c.w_avg[0] / wn) * std::sin(0.5 * wn);
+            const double qj = (c.w_avg[1] / wn) * std::sin(0.5 * wn);
+            const double qk = (c.w_avg[2] / wn) * std::sin(0.5 * wn);
+            const double t = (q0 * q0) - 0.5;
+            const double nx_ = c.pos[0] + 2 * ((t + (qi * qi)) * v[0] + ((qi * qj) - (q0 * qk)) * v[1] + ((qi * qk) + (q0 * qj)) * v[2]);
+            const double ny_ = c.pos[1] + 2 * (((qi * qj) + (q0 * qk)) * v[0] + (t + (qj * qj)) * v[1] + ((qj * qk) - (q0 * qi)) * v[2]);
+            const double nz_ = c.pos[2] + 2 * (((qi * qj) - (q0 * qj)) * v[0] + ((qj * qk) + (q0 * qi)) * v[1] + (t + (qk * qk)) * v[2]);
+            const double np3[3] = {nx_, ny_, nz_};
+            for (int a = 0; a < 3; a++) *p3[a] = periodic(a) ? wrap_pos(a, np3[a]) : np3[a];
+        }
+    }
+
+    void reset_node_forces() {
+        std::fill(nfx.begin(), nfx.end(), 0.0); std::fill(nfy.begin(), nfy.end(), 0.0); std::fill(nfz.begin(), nfz.end(), 0.0);
+        std::fill(ndx.begin(), ndx.end(), 0.0); std::fill(ndy.begin(), ndy.end(), 0.0); std::fill(ndz.begin(), ndz.end(), 0.0);
+    }
+
+    // immersedBoundaryMethod (ibm.cu:6-192)
+    void stage(int step) {
+        old_values();
+        update_macr();
+        reset_node_forces();
+        collisions(step);
+        update_velocity();
+        for (int it = 0; it < d.max_iterations; it++) {
+            interp_spread();
+            update_velocity();
+            euler_sum();
+        }
+        movement();
+        node_movement();
+    }
+};
+
 struct Handle {
     int prec;
     void* lat;
+    Ibm* ibm = nullptr;
 };
 
 } // namespace
@@ -1111,6 +1676,7 @@ void* or_create(const or_desc* d, int prec) {
 void or_destroy(void* hv) {
     Handle* h = (Handle*)hv;
     if (!h) return;
+    delete h->ibm;
     if (h->prec == 4) delete (Lattice<float>*)h->lat;
     else delete (Lattice<double>*)h->lat;
     delete h;
@@ -1190,5 +1756,233 @@ void or_apply_bc_raw(void* h, double* post_stream, const double* post_col) {
         for (size_t k = 0; k < L->pop.size(); k++) post_stream[k] = (double)L->popAux[k];
     });
 }
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Immersed boundary entry points (FP64 lattices only)
+// ---------------------------------------------------------------------------
+namespace {
+
+// makeSpherePolar (src/CUDA/IBM/structs/particle.cpp:698-995): layered polar
+// mesh, equal-area rings, optional Coulomb relaxation, shifted to the centre.
+int sphere_polar(double diameter, const double* center, double mesh_scale, double breugem,
+                 int coulomb, double* xyz, double* S_out, int cap) {
+    double r = diameter / 2.0;
+    const double S_total = 4.0 * M_PI * r * r;
+    r -= breugem;
+    const unsigned nLayer = (unsigned)(2.0 * std::sqrt(2) * r / mesh_scale + 1.0);
+    std::vector<unsigned> nNodes(nLayer + 1);
+    std::vector<double> theta(nLayer + 1), zeta(nLayer + 1), S(nLayer + 1);
+    unsigned total = 0;
+    for (unsigned i = 0; i <= nLayer; i++) {
+        theta[i] = M_PI * ((double)i / (double)nLayer - 0.5);
+        nNodes[i] = (unsigned)(1.5 + std::cos(theta[i]) * nLayer * std::sqrt(3));
+        total += nNodes[i];
+        zeta[i] = r * std::sin(theta[i]);
+    }
+    if (xyz == nullptr) return (int)total;
+    if ((int)total > cap) return -1;
+    for (unsigned i = 0; i < nLayer; i++) S[i] = (zeta[i] + zeta[i + 1]) / 2.0 - zeta[0];
+    S[nLayer] = 2 * r;
+    for (unsigned i = 0; i <= nLayer; i++) S[i] = 2 * M_PI * r * S[i];
+    for (unsigned i = nLayer; i > 0; i--) S[i] = S[i] - S[i - 1];
+    S[0] = S[nLayer];
+    std::vector<double> P(3 * (size_t)total, 0.0), A(total, 0.0);
+    P[0] = 0; P[1] = 0; P[2] = r * std::sin(theta[0]); A[0] = S[0];
+    unsigned node = 1;
+    double phase = 0.0;
+    for (unsigned i = 1; i < nLayer; i++) {
+        if (i % 2 == 1) phase = phase + M_PI / nNodes[i];
+        for (unsigned j = 0; j < nNodes[i]; j++) {
+            P[3 * node + 0] = r * std::cos(theta[i]) * std::cos((double)j * 2.0 * M_PI / nNodes[i] + phase);
+            P[3 * node + 1] = r * std::cos(theta[i]) * std::sin((double)j * 2.0 * M_PI / nNodes[i] + phase);
+            P[3 * node + 2] = r * std::sin(theta[i]);
+            A[node] = S[i] / nNodes[i];
+            node++;
+        }
+    }
+    // north pole (the pole layers hold one node each)
+    P[3 * (total - 1) + 0] = 0; P[3 * (total - 1) + 1] = 0;
+    P[3 * (total - 1) + 2] = r * std::sin(theta[nLayer]);
+    A[total - 1] = S[nLayer];
+    if (coulomb != 0) {
+        std::vector<double> cf(3 * (size_t)total);
+        const double scaleF = 0.001;
+        for (int c = 0; c < coulomb; c++) {
+            std::fill(cf.begin(), cf.end(), 0.0);
+            for (unsigned i = 0; i < total; i++)
+                for (unsigned j = i + 1; j < total; j++) {
+                    const double dx = P[3 * j] - P[3 * i], dy = P[3 * j + 1] - P[3 * i + 1], dz = P[3 * j + 2] - P[3 * i + 2];
+                    const double mag = dx * dx + dy * dy + dz * dz;
+                    cf[3 * i] -= dx / mag; cf[3 * i + 1] -= dy / mag; cf[3 * i + 2] -= dz / mag;
+                    cf[3 * j] -= -dx / mag; cf[3 * j + 1] -= -dy / mag; cf[3 * j + 2] -= -dz / mag;
+                }
+            for (unsigned i = 0; i < total; i++) {
+                for (int a = 0; a < 3; a++) P[3 * i + a] += cf[3 * i + a] / scaleF;
+                const double mag = std::sqrt(P[3 * i] * P[3 * i] + P[3 * i + 1] * P[3 * i + 1] + P[3 * i + 2] * P[3 * i + 2]);
+                for (int a = 0; a < 3; a++) P[3 * i + a] *= r / mag;
+            }
+        }
+        for (unsigned i = 0; i < total; i++) A[i] = S_total / total;
+    }
+    for (unsigned i = 0; i < total; i++) {
+        for (int a = 0; a < 3; a++) xyz[3 * i + a] = P[3 * i + a] + center[a];
+        S_out[i] = A[i];
+    }
+    return (int)total;
+}
+
+} // namespace
+
+extern "C" {
+
+int or_sphere_polar(double diameter, const double* center, double mesh_scale, double breugem, int coulomb,
+                    double* xyz, double* S, int cap) {
+    return sphere_polar(diameter, center, mesh_scale, breugem, coulomb, xyz, S, cap);
+}
+
+int or_ibm_enable(void* hv, const or_ibm_desc* d) {
+    Handle* h = (Handle*)hv;
+    if (h->prec != 8 || h->ibm) return -1;
+    h->ibm = new Ibm((Lattice<double>*)h->lat, *d);
+    return 0;
+}
+
+// state: pos[3], vel[3], w[3]; nodes: absolute positions xyz[3n], areas S[n]
+int or_ibm_add_particle(void* hv, const double* state, double radius, double density, int movable,
+                        int n_nodes, const double* xyz, const double* S) {
+    Handle* h = (Handle*)hv;
+    if (!h->ibm) return -1;
+    Ibm& I = *h->ibm;
+    PCenter c{};
+    for (int a = 0; a < 3; a++) {
+        c.pos[a] = c.pos_old[a] = state[a];
+        c.vel[a] = c.vel_old[a] = state[3 + a];
+        c.w[a] = c.w_avg[a] = c.w_old[a] = state[6 + a];
+    }
+    c.radius = radius;
+    c.volume = radius * radius * radius * 4 * M_PI / 3;
+    c.S = 4.0 * M_PI * radius * radius;
+    c.density = density;
+    const double Ival = 2.0 * c.volume * density * radius * radius / 5.0;
+    c.I[0] = c.I[1] = c.I[2] = Ival;
+    c.movable = movable != 0;
+    for (int i = 0; i < kMaxColl; i++) { c.partner[i] = -1; c.last[i] = -1; }
+    const int id = (int)I.pc.size();
+    I.pc.push_back(c);
+    for (int k = 0; k < n_nodes; k++) {
+        I.px.push_back(xyz[3 * k]); I.py.push_back(xyz[3 * k + 1]); I.pz.push_back(xyz[3 * k + 2]);
+        I.nS.push_back(S[k]); I.nc.push_back(id);
+        I.nfx.push_back(0); I.nfy.push_back(0); I.nfz.push_back(0);
+        I.ndx.push_back(0); I.ndy.push_back(0); I.ndz.push_back(0);
+    }
+    return id;
+}
+
+// n coupled steps: LBM step then the IBM stage labelled first_step + s
+void or_ibm_run(void* hv, int nsteps, int first_step, int nthreads) {
+    Handle* h = (Handle*)hv;
+    Lattice<double>* L = (Lattice<double>*)h->lat;
+    for (int s = 0; s < nsteps; s++) {
+        L->step(false, nthreads);
+        h->ibm->stage(first_step + s);
+    }
+}
+
+void or_ibm_stage(void* hv, int step) { ((Handle*)hv)->ibm->stage(step); }
+void or_ibm_collisions(void* hv, int step) { ((Handle*)hv)->ibm->collisions(step); }
+void or_ibm_update_velocity(void* hv) { ((Handle*)hv)->ibm->update_velocity(); }
+
+int or_ibm_count(void* hv, int* n_nodes) {
+    Handle* h = (Handle*)hv;
+    if (n_nodes) *n_nodes = (int)h->ibm->px.size();
+    return (int)h->ibm->pc.size();
+}
+
+// 48 doubles per particle: pos vel w f M I[6] pos_old vel_old w_old w_avg
+// w_pos f_old M_old dP dL (3 each unless noted)
+void or_ibm_get_particles(void* hv, double* out) {
+    Ibm& I = *((Handle*)hv)->ibm;
+    for (size_t p = 0; p < I.pc.size(); p++) {
+        const PCenter& c = I.pc[p];
+        double* o = out + 48 * p;
+        const double* src[] = {c.pos, c.vel, c.w, c.f, c.M};
+        int k = 0;
+        for (auto s : src) for (int a = 0; a < 3; a++) o[k++] = s[a];
+        for (int a = 0; a < 6; a++) o[k++] = c.I[a];
+        const double* src2[] = {c.pos_old, c.vel_old, c.w_old, c.w_avg, c.w_pos, c.f_old, c.M_old, c.dP, c.dL};
+        for (auto s : src2) for (int a = 0; a < 3; a++) o[k++] = s[a];
+    }
+}
+
+void or_ibm_set_particles(void* hv, const double* in) {
+    Ibm& I = *((Handle*)hv)->ibm;
+    for (size_t p = 0; p < I.pc.size(); p++) {
+        PCenter& c = I.pc[p];
+        const double* o = in + 48 * p;
+        double* dst[] = {c.pos, c.vel, c.w, c.f, c.M};
+        int k = 0;
+        for (auto s : dst) for (int a = 0; a < 3; a++) s[a] = o[k++];
+        for (int a = 0; a < 6; a++) c.I[a] = o[k++];
+        double* dst2[] = {c.pos_old, c.vel_old, c.w_old, c.w_avg, c.w_pos, c.f_old, c.M_old, c.dP, c.dL};
+        for (auto s : dst2) for (int a = 0; a < 3; a++) s[a] = o[k++];
+    }
+}
+
+void or_ibm_get_collisions(void* hv, int* partner, double* tang, int* last) {
+    Ibm& I = *((Handle*)hv)->ibm;
+    for (size_t p = 0; p < I.pc.size(); p++)
+        for (int i = 0; i < kMaxColl; i++) {
+            partner[p * kMaxColl + i] = I.pc[p].partner[i];
+            last[p * kMaxColl + i] = I.pc[p].last[i];
+            for (int a = 0; a < 3; a++) tang[(p * kMaxColl + i) * 3 + a] = I.pc[p].tang[i][a];
+        }
+}
+
+// node positions, accumulated forces and last force increments, [3][n] each
+void or_ibm_get_nodes(void* hv, double* pos, double* f, double* df) {
+    Ibm& I = *((Handle*)hv)->ibm;
+    const size_t n = I.px.size();
+    for (size_t k = 0; k < n; k++) {
+        pos[k] = I.px[k]; pos[n + k] = I.py[k]; pos[2 * n + k] = I.pz[k];
+        f[k] = I.nfx[k]; f[n + k] = I.nfy[k]; f[2 * n + k] = I.nfz[k];
+        df[k] = I.ndx[k]; df[n + k] = I.ndy[k]; df[2 * n + k] = I.ndz[k];
+    }
+}
+
+// per-node lattice force field (what the next collision reads), [3][nodes]
+void or_ibm_get_force(void* hv, double* f) {
+    Handle* h = (Handle*)hv;
+    Lattice<double>* L = (Lattice<double>*)h->lat;
+    const size_t n = L->nnodes;
+    for (size_t k = 0; k < n; k++) { f[k] = L->force_x[k]; f[n + k] = L->force_y[k]; f[2 * n + k] = L->force_z[k]; }
+}
+
+void or_ibm_set_force(void* hv, const double* f) {
+    Handle* h = (Handle*)hv;
+    Lattice<double>* L = (Lattice<double>*)h->lat;
+    const size_t n = L->nnodes;
+    for (size_t k = 0; k < n; k++) { L->force_x[k] = f[k]; L->force_y[k] = f[n + k]; L->force_z[k] = f[2 * n + k]; }
+}
+
+// one interpolation/spread pass on the current macroscopics; returns the
+// auxiliary fields [6][nodes] (velAux xyz, fAux xyz) without summing them
+void or_ibm_interp_spread(void* hv, double* aux_out) {
+    Ibm& I = *((Handle*)hv)->ibm;
+    I.interp_spread();
+    const size_t n = I.L->nnodes;
+    for (int a = 0; a < 3; a++)
+        for (size_t k = 0; k < n; k++) { aux_out[a * n + k] = I.velAux[a][k]; aux_out[(3 + a) * n + k] = I.fAux[a][k]; }
+}
+
+void or_ibm_update_macr(void* hv) { ((Handle*)hv)->ibm->update_macr(); }
+
+void or_ibm_set_macr(void* hv, const double* rho, const double* ux, const double* uy, const double* uz) {
+    Lattice<double>* L = (Lattice<double>*)((Handle*)hv)->lat;
+    for (size_t k = 0; k < L->nnodes; k++) { L->rho[k] = rho[k]; L->ux[k] = ux[k]; L->uy[k] = uy[k]; L->uz[k] = uz[k]; }
+}
+
+double or_stencil4(double x) { return stencil4(x); }
 
 } // extern "C"

@@ -38,6 +38,32 @@ class OrDesc(C.Structure):
     ]
 
 
+class OrIbmDesc(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_int),
+        ("periodic_x", C.c_int), ("periodic_y", C.c_int), ("periodic_z", C.c_int),
+        ("thickness", C.c_double), ("alpha", C.c_double), ("fluid_density", C.c_double),
+        ("gx", C.c_double), ("gy", C.c_double), ("gz", C.c_double),
+        ("pp_friction", C.c_double), ("pw_friction", C.c_double),
+        ("pp_rest", C.c_double), ("pw_rest", C.c_double),
+        ("young", C.c_double), ("poisson", C.c_double),
+        ("wall_young", C.c_double), ("wall_poisson", C.c_double),
+        ("ref_float_helpers", C.c_int),
+    ]
+
+
+# record of one particle returned by Oracle.ibm_particles(): name -> slice
+PARTICLE_FIELDS = {}
+_off = 0
+for _name, _n in (("pos", 3), ("vel", 3), ("w", 3), ("f", 3), ("M", 3), ("I", 6), ("pos_old", 3),
+                  ("vel_old", 3), ("w_old", 3), ("w_avg", 3), ("w_pos", 3), ("f_old", 3), ("M_old", 3),
+                  ("dP", 3), ("dL", 3)):
+    PARTICLE_FIELDS[_name] = slice(_off, _off + _n)
+    _off += _n
+PARTICLE_RECORD = _off  # 48
+MAX_COLLISIONS = 13
+
+
 _lib = None
 
 
@@ -67,6 +93,30 @@ def lib():
         L.or_set_omega.argtypes = [C.c_void_p, dp]
         L.or_collide_node.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
         L.or_apply_bc_raw.argtypes = [C.c_void_p, dp, dp]
+        ip = P(C.c_int)
+        L.or_sphere_polar.restype = C.c_int
+        L.or_sphere_polar.argtypes = [C.c_double, dp, C.c_double, C.c_double, C.c_int, dp, dp, C.c_int]
+        L.or_ibm_enable.restype = C.c_int
+        L.or_ibm_enable.argtypes = [C.c_void_p, P(OrIbmDesc)]
+        L.or_ibm_add_particle.restype = C.c_int
+        L.or_ibm_add_particle.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp]
+        L.or_ibm_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.or_ibm_stage.argtypes = [C.c_void_p, C.c_int]
+        L.or_ibm_collisions.argtypes = [C.c_void_p, C.c_int]
+        L.or_ibm_update_velocity.argtypes = [C.c_void_p]
+        L.or_ibm_count.restype = C.c_int
+        L.or_ibm_count.argtypes = [C.c_void_p, ip]
+        L.or_ibm_get_particles.argtypes = [C.c_void_p, dp]
+        L.or_ibm_set_particles.argtypes = [C.c_void_p, dp]
+        L.or_ibm_get_collisions.argtypes = [C.c_void_p, ip, dp, ip]
+        L.or_ibm_get_nodes.argtypes = [C.c_void_p, dp, dp, dp]
+        L.or_ibm_get_force.argtypes = [C.c_void_p, dp]
+        L.or_ibm_set_force.argtypes = [C.c_void_p, dp]
+        L.or_ibm_interp_spread.argtypes = [C.c_void_p, dp]
+        L.or_ibm_update_macr.argtypes = [C.c_void_p]
+        L.or_ibm_set_macr.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_stencil4.restype = C.c_double
+        L.or_stencil4.argtypes = [C.c_double]
         _lib = L
     return _lib
 
@@ -157,6 +207,83 @@ class Oracle:
         lib().or_collide_node(self.h, _p(f), fx, fy, fz, omega_old, _p(out), _p(m))
         return out, m
 
+    # ---- immersed boundary stage (FP64 lattices) ----
+    def ibm_enable(self, idesc: "OrIbmDesc"):
+        if lib().or_ibm_enable(self.h, C.byref(idesc)) != 0:
+            raise ValueError("IBM needs an FP64 oracle lattice")
+
+    def ibm_add_particle(self, pos, vel, w, radius, density, movable, xyz, S) -> int:
+        state = np.ascontiguousarray(np.concatenate([pos, vel, w]), dtype=np.float64)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        S = np.ascontiguousarray(S, dtype=np.float64)
+        return lib().or_ibm_add_particle(self.h, _p(state), float(radius), float(density), int(bool(movable)),
+                                         len(S), _p(xyz), _p(S))
+
+    def ibm_run(self, n, first_step, threads=0):
+        lib().or_ibm_run(self.h, int(n), int(first_step), int(threads))
+
+    def ibm_stage(self, step):
+        lib().or_ibm_stage(self.h, int(step))
+
+    def ibm_collisions(self, step):
+        lib().or_ibm_collisions(self.h, int(step))
+
+    def ibm_update_velocity(self):
+        lib().or_ibm_update_velocity(self.h)
+
+    def ibm_counts(self):
+        n = C.c_int(0)
+        np_ = lib().or_ibm_count(self.h, C.byref(n))
+        return np_, n.value
+
+    def ibm_particles(self) -> np.ndarray:
+        np_, _ = self.ibm_counts()
+        out = np.zeros((np_, PARTICLE_RECORD))
+        if np_:
+            lib().or_ibm_get_particles(self.h, _p(out))
+        return out
+
+    def ibm_set_particles(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=np.float64)
+        lib().or_ibm_set_particles(self.h, _p(rec))
+
+    def ibm_collision_state(self):
+        np_, _ = self.ibm_counts()
+        partner = np.zeros((np_, MAX_COLLISIONS), dtype=np.int32)
+        last = np.zeros((np_, MAX_COLLISIONS), dtype=np.int32)
+        tang = np.zeros((np_, MAX_COLLISIONS, 3))
+        if np_:
+            lib().or_ibm_get_collisions(self.h, partner.ctypes.data_as(C.POINTER(C.c_int)), _p(tang),
+                                        last.ctypes.data_as(C.POINTER(C.c_int)))
+        return partner, tang, last
+
+    def ibm_nodes(self):
+        _, n = self.ibm_counts()
+        pos, f, df = np.zeros((3, n)), np.zeros((3, n)), np.zeros((3, n))
+        if n:
+            lib().or_ibm_get_nodes(self.h, _p(pos), _p(f), _p(df))
+        return pos, f, df
+
+    def ibm_force(self) -> np.ndarray:
+        out = np.zeros((3,) + self.shape)
+        lib().or_ibm_get_force(self.h, _p(out))
+        return out
+
+    def ibm_set_force(self, f):
+        lib().or_ibm_set_force(self.h, _p(np.ascontiguousarray(f, dtype=np.float64)))
+
+    def ibm_interp_spread(self) -> np.ndarray:
+        out = np.zeros((6,) + self.shape)
+        lib().or_ibm_interp_spread(self.h, _p(out))
+        return out
+
+    def ibm_update_macr(self):
+        lib().or_ibm_update_macr(self.h)
+
+    def ibm_set_macr(self, rho, ux, uy, uz):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, ux, uy, uz)]
+        lib().or_ibm_set_macr(self.h, *[_p(a) for a in arrs])
+
     def apply_bc_raw(self, post_stream, post_col):
         ps = np.ascontiguousarray(post_stream, dtype=np.float64).copy()
         pc = np.ascontiguousarray(post_col, dtype=np.float64)
@@ -166,3 +293,37 @@ class Oracle:
 
 def max_threads() -> int:
     return lib().or_max_threads()
+
+
+def ibm_desc(**kw) -> OrIbmDesc:
+    """Oracle IBM descriptor; defaults are the reference's (src/CUDA/IBM/ibmVar.h,
+    src/CUDA/IBM/ibmBoundaryCondition.h)."""
+    d = OrIbmDesc(max_iterations=1, periodic_x=1, periodic_y=0, periodic_z=1, thickness=1.0, alpha=0.5,
+                  fluid_density=1.0, gx=0.0, gy=0.0, gz=0.0, pp_friction=0.1, pw_friction=0.1, pp_rest=0.9,
+                  pw_rest=0.9, young=0.5, poisson=0.3, wall_young=100.0, wall_poisson=0.3, ref_float_helpers=1)
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+def ibm_desc_from(vdesc) -> OrIbmDesc:
+    """Oracle IBM descriptor with the same settings as a product vlbm_ibm_desc."""
+    d = OrIbmDesc()
+    for name, _ in OrIbmDesc._fields_:
+        setattr(d, name, getattr(vdesc, name))
+    return d
+
+
+def sphere_polar(diameter, center, mesh_scale=1.0, breugem=0.0, coulomb=0):
+    """Layered polar sphere mesh of the reference: (xyz [n,3], S [n])."""
+    c = np.ascontiguousarray(center, dtype=np.float64)
+    n = lib().or_sphere_polar(float(diameter), _p(c), float(mesh_scale), float(breugem), int(coulomb),
+                              None, None, 0)
+    xyz, S = np.zeros((n, 3)), np.zeros(n)
+    lib().or_sphere_polar(float(diameter), _p(c), float(mesh_scale), float(breugem), int(coulomb),
+                          _p(xyz), _p(S), n)
+    return xyz, S
+
+
+def stencil4(x: float) -> float:
+    return lib().or_stencil4(float(x))
