@@ -162,6 +162,102 @@ int vlbm_sync(vlbm_h h);
 /* Device bytes held by the handle. */
 int vlbm_device_bytes(vlbm_h h, size_t* bytes);
 
+/* ------------------------------------------------------------------------
+ * Immersed boundary stage for rigid spheres (src/CUDA/IBM/). When enabled,
+ * every vlbm_step runs the LBM step and then the stage of
+ * immersedBoundaryMethod (src/CUDA/IBM/ibm.cu:6-192), labelled with the
+ * number of steps done before it, as main.cu:369-389 does with its loop
+ * counter. Particle arithmetic is FP64 whatever the lattice precision.
+ * ------------------------------------------------------------------------ */
+
+/* sphere meshes for vlbm_ibm_add_sphere */
+#define VLBM_MESH_POLAR 0        /* Particle::makeSpherePolar (src/CUDA/IBM/structs/particle.cpp:698-995) */
+#define VLBM_MESH_FIBONACCI 1    /* Fibonacci sphere lattice of n equal-area nodes (no reference counterpart) */
+
+/* Particle record of vlbm_ibm_get_particles / vlbm_ibm_set_particles: 48
+ * doubles per sphere, in this order (ParticleCenter,
+ * src/CUDA/IBM/structs/particleCenter.h:65-118): pos[3] vel[3] w[3] f[3]
+ * M[3] I[6] (xx yy zz xy xz yz) pos_old[3] vel_old[3] w_old[3] w_avg[3]
+ * w_pos[3] f_old[3] M_old[3] dP_internal[3] dL_internal[3]. */
+#define VLBM_IBM_RECORD 48
+#define VLBM_IBM_HISTORY 13      /* MAX_ACTIVE_COLLISIONS: 7 wall slots + 6 partner slots */
+
+/* Runtime form of src/CUDA/IBM/ibmVar.h and src/CUDA/IBM/ibmBoundaryCondition.h. */
+typedef struct vlbm_ibm_desc {
+    int max_iterations;                       /* IBM_MAX_ITERATION */
+    int periodic_x, periodic_y, periodic_z;   /* IBM_BC_{X,Y,Z}_PERIODIC; 0 = IBM_BC_*_WALL (walls at 0 and N-1) */
+    double thickness;                         /* IBM_THICKNESS */
+    double alpha;                             /* IBM_MOVEMENT_DISCRETIZATION */
+    double fluid_density;                     /* FLUID_DENSITY */
+    double gx, gy, gz;                        /* GX, GY, GZ */
+    double pp_friction, pw_friction;          /* PP_FRICTION_COEF, PW_FRICTION_COEF */
+    double pp_rest, pw_rest;                  /* PP_REST_COEF, PW_REST_COEF */
+    double young, poisson;                    /* PARTICLE_YOUNG_MODULUS, PARTICLE_POISSON_RATIO */
+    double wall_young, wall_poisson;          /* WALL_YOUNG_MODULUS, WALL_POISSON_RATIO */
+    int ref_float_helpers;                    /* 1: sqrtf / rsqrtf / fabsf where the FP64 reference calls
+                                                 them (vector_length, vector_normalize, sphereSphereGap,
+                                                 sphereWallCollision); 0: FP64 throughout */
+    double mesh_scale;                        /* MESH_SCALE (polar mesh) */
+    double breugem;                           /* BREUGEM_PARAMETER (polar mesh) */
+    int coulomb;                              /* MESH_COULOMB (polar mesh) */
+    int max_spheres, max_nodes;               /* capacities allocated by vlbm_ibm_enable */
+} vlbm_ibm_desc;
+
+/* Reference defaults (NUM_PARTICLES-independent constants of ibmVar.h /
+ * ibmBoundaryCondition.h); capacities 256 spheres, 262144 nodes. */
+void vlbm_ibm_desc_defaults(vlbm_ibm_desc* d);
+
+/* Allocates the particle arrays (ParticlesSoA, src/CUDA/IBM/structs/particle.cpp:5-63),
+ * the per-node force field (macr.f) and the spread-force buffer (IBMMacrsAux,
+ * src/CUDA/IBM/structs/ibmMacrsAux.h) and switches the handle to IBM steps. */
+int vlbm_ibm_enable(vlbm_h h, const vlbm_ibm_desc* d);
+
+/* Adds a sphere (makeSpherePolar / a Fibonacci lattice) with state
+ * {pos[3], vel[3], w[3]}; n_fibonacci is the node count of VLBM_MESH_FIBONACCI.
+ * I = 2/5 m r^2, history slots cleared. *id receives the sphere index. */
+int vlbm_ibm_add_sphere(vlbm_h h, const double state[9], double radius, double density, int movable,
+                        int mesh, int n_fibonacci, int* id);
+
+/* Adds a sphere whose Lagrangian nodes are given: positions xyz[3*n]
+ * (x0 y0 z0 x1 ...), areas area[n] (ParticleNode pos / S). */
+int vlbm_ibm_add_particle(vlbm_h h, const double state[9], double radius, double density, int movable,
+                          int n_nodes, const double* xyz, const double* area, int* id);
+
+int vlbm_ibm_counts(vlbm_h h, int* n_spheres, int* n_nodes);
+
+/* Particle records (VLBM_IBM_RECORD doubles each), host <-> device. */
+int vlbm_ibm_get_particles(vlbm_h h, double* out);
+int vlbm_ibm_set_particles(vlbm_h h, const double* in);
+
+/* Contact history (CollisionData, src/CUDA/IBM/structs/particleCenter.h:36-63):
+ * partner[n][13], tang[n][13][3], last[n][13]; any pointer may be NULL. */
+int vlbm_ibm_get_collisions(vlbm_h h, int* partner, double* tang, int* last);
+
+/* Lagrangian nodes: positions, accumulated forces of the last stage and the
+ * force of its last iteration (ParticleNodeSoA pos / f / deltaF), [3][n] each. */
+int vlbm_ibm_get_nodes(vlbm_h h, double* pos, double* force, double* dforce);
+
+/* Force field the next collision uses, [3][NZ][NY][NX] doubles (macr.f). */
+int vlbm_ibm_get_force(vlbm_h h, double* f);
+
+/* Uniform cell list of the last stage: dims[3] cells per axis and the cell
+ * index of each sphere (x fastest). No reference counterpart: the reference
+ * tests all pairs (gpuParticlesCollisionHandler, ibmCollision.cu:2381-2448). */
+int vlbm_ibm_cells(vlbm_h h, int* dims, int* cell);
+
+/* Runs one IBM stage on the current state without an LBM step, labelled like
+ * the next step would be; advances the step counter (for tests and for
+ * particles without fluid coupling). */
+int vlbm_ibm_stage(vlbm_h h);
+
+/* VLBM_OK, or VLBM_ERR_UNSUPPORTED when a sphere had more contacts than the
+ * contact buffer holds (16 per sphere and step). */
+int vlbm_ibm_status(vlbm_h h);
+
+/* Runs n_steps and reports the device time of the LBM step kernels and of the
+ * IBM stages separately (HIP events around each part). */
+int vlbm_ibm_time_steps(vlbm_h h, int n_steps, double* ms_lbm, double* ms_ibm);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1985,4 +1985,26 @@ void or_ibm_set_macr(void* hv, const double* rho, const double* ux, const double
 
 double or_stencil4(double x) { return stencil4(x); }
 
+// Cell of each sphere in the uniform cell list used by the product's contact
+// search (no reference counterpart: the reference tests all pairs). Cells are
+// at least 2 r_max (1 + 1e-4) wide, index floor(p / width) clamped to the grid.
+void or_ibm_cells(int nx, int ny, int nz, double r_max, const double* pos, int n, int* dims, int* cells) {
+    const int N[3] = {nx, ny, nz};
+    const double cs = 2.0 * r_max * 1.0001;
+    double w[3];
+    for (int a = 0; a < 3; a++) {
+        int c = cs > 0 ? (int)(N[a] / cs) : 1;
+        dims[a] = c < 1 ? 1 : c;
+        w[a] = (double)N[a] / dims[a];
+    }
+    for (int i = 0; i < n; i++) {
+        int k[3];
+        for (int a = 0; a < 3; a++) {
+            int v = (int)std::floor(pos[3 * i + a] / w[a]);
+            k[a] = v < 0 ? 0 : (v >= dims[a] ? dims[a] - 1 : v);
+        }
+        cells[i] = k[0] + dims[0] * (k[1] + dims[1] * k[2]);
+    }
+}
+
 } // extern "C"

@@ -115,6 +115,7 @@ def lib():
         L.or_ibm_interp_spread.argtypes = [C.c_void_p, dp]
         L.or_ibm_update_macr.argtypes = [C.c_void_p]
         L.or_ibm_set_macr.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_ibm_cells.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, dp, C.c_int, ip, ip]
         L.or_stencil4.restype = C.c_double
         L.or_stencil4.argtypes = [C.c_double]
         _lib = L
@@ -323,6 +324,17 @@ def sphere_polar(diameter, center, mesh_scale=1.0, breugem=0.0, coulomb=0):
     lib().or_sphere_polar(float(diameter), _p(c), float(mesh_scale), float(breugem), int(coulomb),
                           _p(xyz), _p(S), n)
     return xyz, S
+
+
+def ibm_cells(shape_xyz, r_max, pos):
+    """(dims[3], cell[n]) of the uniform contact cell list for positions pos[n, 3]."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    dims = np.zeros(3, dtype=np.int32)
+    cells = np.zeros(max(len(pos), 1), dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    lib().or_ibm_cells(int(shape_xyz[0]), int(shape_xyz[1]), int(shape_xyz[2]), float(r_max), _p(pos), len(pos),
+                       dims.ctypes.data_as(ip), cells.ctypes.data_as(ip))
+    return dims, cells[:len(pos)]
 
 
 def stencil4(x: float) -> float:

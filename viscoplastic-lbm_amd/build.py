@@ -34,6 +34,8 @@ SOURCES = [
     "kernels_fp32.hip",
     "kernels_fp16.hip",
     "kernels_common.hip",
+    "kernels_ibm.hip",
+    "ibm_capi.cpp",
     "capi.cpp",
 ]
 

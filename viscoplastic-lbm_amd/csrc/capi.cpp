@@ -14,24 +14,20 @@
 
 #include "../../include/vlbm.h"
 #include "lattice.h"
+#include "ctx.h"
 
 using namespace vlbm;
+using namespace vlbm::detail;
+
+namespace vlbm {
+namespace detail {
+thread_local std::string g_err;
+}  // namespace detail
+}  // namespace vlbm
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                        \
-    do {                                                                                     \
-        hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(VLBM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+// (error state and helpers: ctx.h)
 
 // ---------------------------------------------------------------------------
 // Boundary-map builders.
@@ -238,145 +234,6 @@ int build_words(const vlbm_desc& d, std::vector<uint32_t>& out) {
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// Handle
-// ---------------------------------------------------------------------------
-struct vlbm_ctx {
-    vlbm_desc d;
-    size_t n = 0;             // nodes
-    int64_t stride = 0;       // elements between population planes
-    int sbytes = 0;           // storage bytes per population
-    int tbytes = 0;           // compute / macroscopic bytes per value
-    void* lat[2] = {nullptr, nullptr};
-    int cur = 0;              // lattice holding the current state
-    bool direct = true;       // current state is post-stream (after init / load)
-    long long steps = 0;
-    uint32_t* d_mask = nullptr;
-    std::vector<uint32_t> words;   // host map
-    uint32_t region_word[27];
-    bool mask_geom = false;
-    void* omega_state = nullptr;
-    void* macr[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // rho, ux, uy, uz, omega
-    void* uold[3] = {nullptr, nullptr, nullptr};
-    double* partials = nullptr;
-    double* out3 = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    size_t device_bytes = 0;
-};
-
-namespace {
-
-template <typename T>
-void fill_params(const vlbm_ctx* h, Params<T>& p) {
-    std::memset(&p, 0, sizeof(p));
-    const vlbm_desc& d = h->d;
-    p.nx = d.nx; p.ny = d.ny; p.nz = d.nz;
-    p.n = (uint32_t)h->n;
-    p.div_nx = make_fastdiv((uint32_t)d.nx);
-    p.div_nxy = make_fastdiv((uint32_t)(d.nx * d.ny));
-    p.stride = h->stride;
-    p.mask = h->d_mask;
-    for (int r = 0; r < 27; r++) p.region_word[r] = h->region_word[r];
-    for (int k = 0; k < 8; k++) {
-        p.ux_bc[k] = (T)d.ux_bc[k]; p.uy_bc[k] = (T)d.uy_bc[k];
-        p.uz_bc[k] = (T)d.uz_bc[k]; p.rho_bc[k] = (T)d.rho_bc[k];
-    }
-    p.fx = (T)d.fx; p.fy = (T)d.fy; p.fz = (T)d.fz;
-    p.omega = (T)(1.0 / d.tau);
-    p.omega_p = (T)(1.0 / (3.0 * d.eta_p + 0.5));
-    p.s_y = (T)d.s_y;
-    p.k_cons = (T)d.k_consistency;
-    p.n_minus_1 = (T)(d.n_index - 1.0);
-    p.gamma_0 = (T)d.gamma_0;
-    p.clamp = d.omega_clamp;
-    if (d.omega_clamp) { p.omega_lo = (T)(1.0 / d.tau_max); p.omega_hi = (T)(1.0 / d.tau_min); }
-    p.omega_state = (T*)h->omega_state;
-    p.m_rho = (T*)h->macr[0]; p.m_ux = (T*)h->macr[1]; p.m_uy = (T*)h->macr[2];
-    p.m_uz = (T*)h->macr[3]; p.m_omega = (T*)h->macr[4];
-    p.zouhe_textbook = d.zouhe_textbook;
-    p.duct_R = (T)d.duct_outer_radius; p.duct_r = (T)d.duct_inner_radius;
-    p.w_out = (T)d.outer_rotation; p.u_out = (T)d.outer_velocity;
-    p.w_in = (T)d.inner_rotation; p.u_in = (T)d.inner_velocity;
-}
-
-// Runs one kernel with parameters of the handle's compute type.
-template <typename F>
-int with_params(vlbm_ctx* h, F&& fn) {
-    if (h->tbytes == 8) {
-        Params<double> p;
-        fill_params(h, p);
-        return fn((void*)&p, p);
-    }
-    Params<float> p;
-    fill_params(h, p);
-    return fn((void*)&p, p);
-}
-
-int check_launch(int rc, const char* what) {
-    if (rc != 0) {
-        hipError_t e = hipGetLastError();
-        return fail(VLBM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-    }
-    return VLBM_OK;
-}
-
-void release(vlbm_ctx* h) {
-    if (!h) return;
-    for (void* p : h->lat) if (p) (void)hipFree(p);
-    for (void* p : h->macr) if (p) (void)hipFree(p);
-    for (void* p : h->uold) if (p) (void)hipFree(p);
-    if (h->omega_state) (void)hipFree(h->omega_state);
-    if (h->d_mask) (void)hipFree(h->d_mask);
-    if (h->partials) (void)hipFree(h->partials);
-    if (h->out3) (void)hipFree(h->out3);
-    if (h->ev0) (void)hipEventDestroy(h->ev0);
-    if (h->ev1) (void)hipEventDestroy(h->ev1);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
-    delete h;
-}
-
-int dalloc(vlbm_ctx* h, void** p, size_t bytes) {
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return fail(VLBM_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
-    }
-    h->device_bytes += bytes;
-    return VLBM_OK;
-}
-
-// Host copy of a macroscopic field (compute type) as doubles.
-int fetch_field(vlbm_ctx* h, const void* dev, double* out) {
-    if (h->tbytes == 8) {
-        HIP_TRY(hipMemcpyAsync(out, dev, h->n * 8, hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        return VLBM_OK;
-    }
-    std::vector<float> tmp(h->n);
-    HIP_TRY(hipMemcpyAsync(tmp.data(), dev, h->n * 4, hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    for (size_t k = 0; k < h->n; k++) out[k] = tmp[k];
-    return VLBM_OK;
-}
-
-int upload_field(vlbm_ctx* h, void* dev, const double* in) {
-    if (h->tbytes == 8) {
-        HIP_TRY(hipMemcpyAsync(dev, in, h->n * 8, hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        return VLBM_OK;
-    }
-    std::vector<float> tmp(h->n);
-    for (size_t k = 0; k < h->n; k++) tmp[k] = (float)in[k];
-    HIP_TRY(hipMemcpyAsync(dev, tmp.data(), h->n * 4, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return VLBM_OK;
-}
-
-double pop_weight(int q, int i) { return q == 19 ? weight<19>(i) : weight<27>(i); }
-
-}  // namespace
-
 extern "C" {
 
 const char* vlbm_last_error(void) { return g_err.c_str(); }
@@ -535,7 +392,7 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
     if (h->omega_state && (rc = upload_field(h, h->omega_state, zero.data()))) return rc;
     h->direct = true;
     h->steps = 0;
-    return VLBM_OK;
+    return ibm_reset_tiles(h);
 }
 
 int vlbm_set_omega(vlbm_h h, const double* omega) {
@@ -569,7 +426,7 @@ int vlbm_load_pops(vlbm_h h, const double* pop) {
     h->cur = 0;
     h->direct = true;
     h->steps = 0;
-    return VLBM_OK;
+    return ibm_reset_tiles(h);
 }
 
 int vlbm_get_pops(vlbm_h h, double* pop) {
@@ -611,21 +468,33 @@ int vlbm_get_pops(vlbm_h h, double* pop) {
     return VLBM_OK;
 }
 
-static int enqueue_steps(vlbm_h h, int n_steps, int save_last) {
+// n_steps steps; with IBM each LBM step is followed by the IBM stage, and
+// ev (2 events per step plus one, or null) records the boundaries of both.
+static int enqueue_steps(vlbm_h h, int n_steps, int save_last, hipEvent_t* ev = nullptr) {
     const vlbm_desc& d = h->d;
+    const bool ibm = h->ibm != nullptr;
+    if (ev) HIP_TRY(hipEventRecord(ev[0], h->stream));
     for (int s = 0; s < n_steps; s++) {
         const int other = 1 - h->cur;
         const bool save = save_last && (s == n_steps - 1);
         int rc = with_params(h, [&](void* pv, auto& p) {
             p.src = h->lat[h->cur];
             p.dst = h->lat[other];
-            p.save = save ? 1 : 0;
-            return check_launch(launch_step(d.q, h->sbytes, d.rheology, h->mask_geom, h->direct, pv, h->stream),
+            // IBM builds take the macroscopics from the stage (save_macr_to_array
+            // is false with IBM, src/CUDA/main.cu:293-297)
+            p.save = (save && !ibm) ? 1 : 0;
+            return check_launch(launch_step(d.q, h->sbytes, d.rheology, h->mask_geom, h->direct, pv, h->stream, ibm),
                                 "step kernel");
         });
         if (rc) return rc;
         h->cur = other;
         h->direct = false;
+        if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], h->stream));
+        if (ibm) {
+            if ((rc = ibm_stage(h))) return rc;
+            if (save && (rc = ibm_fill_macr(h))) return rc;
+        }
+        if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 2], h->stream));
         h->steps++;
     }
     return VLBM_OK;
@@ -650,13 +519,40 @@ int vlbm_time_steps(vlbm_h h, int n_steps, double* ms) {
     return VLBM_OK;
 }
 
+int vlbm_ibm_time_steps(vlbm_h h, int n_steps, double* ms_lbm, double* ms_ibm) {
+    if (!h || !ms_lbm || !ms_ibm) return fail(VLBM_ERR_INVALID, "handle or outputs NULL");
+    if (!h->ibm) return fail(VLBM_ERR_INVALID, "IBM is not enabled on this handle (vlbm_ibm_enable)");
+    if (n_steps < 1) return fail(VLBM_ERR_INVALID, "n_steps must be >= 1");
+    std::vector<hipEvent_t> ev((size_t)2 * n_steps + 1, nullptr);
+    int rc = VLBM_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { rc = fail(VLBM_ERR_DEVICE, "hipEventCreate failed"); break; }
+    if (!rc) rc = enqueue_steps(h, n_steps, 0, ev.data());
+    if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(VLBM_ERR_DEVICE, "event sync failed");
+    double a = 0, b = 0;
+    for (int s = 0; s < n_steps && !rc; s++) {
+        float t0 = 0, t1 = 0;
+        if (hipEventElapsedTime(&t0, ev[2 * s], ev[2 * s + 1]) != hipSuccess ||
+            hipEventElapsedTime(&t1, ev[2 * s + 1], ev[2 * s + 2]) != hipSuccess)
+            rc = fail(VLBM_ERR_DEVICE, "hipEventElapsedTime failed");
+        a += t0;
+        b += t1;
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    *ms_lbm = a;
+    *ms_ibm = b;
+    return rc;
+}
+
 int vlbm_compute_macroscopics(vlbm_h h) {
     if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
     const vlbm_desc& d = h->d;
     return with_params(h, [&](void* pv, auto& p) {
         p.src = h->lat[h->cur];
         p.dst = h->lat[1 - h->cur];
-        p.aux_mode = kAuxMoments;
+        // with IBM the flagged tiles keep the macroscopics of the last stage
+        p.aux_mode = h->ibm ? kAuxMomentsOutsideTiles : kAuxMoments;
         p.direct = h->direct ? 1 : 0;
         return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream), "moments kernel");
     });

@@ -1,0 +1,187 @@
+// Internal state of a lattice handle and helpers shared by the C-ABI
+// translation units (capi.cpp, ibm_capi.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vlbm.h"
+#include "lattice.h"
+
+namespace vlbm {
+struct IbmState;
+namespace detail {
+
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(VLBM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace detail
+}  // namespace vlbm
+
+// ---------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------
+struct vlbm_ctx {
+    vlbm_desc d;
+    size_t n = 0;             // nodes
+    int64_t stride = 0;       // elements between population planes
+    int sbytes = 0;           // storage bytes per population
+    int tbytes = 0;           // compute / macroscopic bytes per value
+    void* lat[2] = {nullptr, nullptr};
+    int cur = 0;              // lattice holding the current state
+    bool direct = true;       // current state is post-stream (after init / load)
+    long long steps = 0;
+    uint32_t* d_mask = nullptr;
+    std::vector<uint32_t> words;   // host map
+    uint32_t region_word[27];
+    bool mask_geom = false;
+    void* omega_state = nullptr;
+    void* macr[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // rho, ux, uy, uz, omega
+    void* uold[3] = {nullptr, nullptr, nullptr};
+    double* partials = nullptr;
+    double* out3 = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t device_bytes = 0;
+    vlbm::IbmState* ibm = nullptr;   // immersed boundary stage (vlbm_ibm_enable)
+    // what the step kernel needs of the IBM stage (null without IBM)
+    const uint8_t* tile_flag = nullptr;
+    void* ibm_force[3] = {nullptr, nullptr, nullptr};
+    int tiles_x = 0, tiles_y = 0;
+};
+
+namespace vlbm {
+namespace detail {
+
+template <typename T>
+inline void fill_params(const vlbm_ctx* h, Params<T>& p) {
+    std::memset(&p, 0, sizeof(p));
+    const vlbm_desc& d = h->d;
+    p.nx = d.nx; p.ny = d.ny; p.nz = d.nz;
+    p.n = (uint32_t)h->n;
+    p.div_nx = make_fastdiv((uint32_t)d.nx);
+    p.div_nxy = make_fastdiv((uint32_t)(d.nx * d.ny));
+    p.stride = h->stride;
+    p.mask = h->d_mask;
+    for (int r = 0; r < 27; r++) p.region_word[r] = h->region_word[r];
+    for (int k = 0; k < 8; k++) {
+        p.ux_bc[k] = (T)d.ux_bc[k]; p.uy_bc[k] = (T)d.uy_bc[k];
+        p.uz_bc[k] = (T)d.uz_bc[k]; p.rho_bc[k] = (T)d.rho_bc[k];
+    }
+    p.fx = (T)d.fx; p.fy = (T)d.fy; p.fz = (T)d.fz;
+    p.omega = (T)(1.0 / d.tau);
+    p.omega_p = (T)(1.0 / (3.0 * d.eta_p + 0.5));
+    p.s_y = (T)d.s_y;
+    p.k_cons = (T)d.k_consistency;
+    p.n_minus_1 = (T)(d.n_index - 1.0);
+    p.gamma_0 = (T)d.gamma_0;
+    p.clamp = d.omega_clamp;
+    if (d.omega_clamp) { p.omega_lo = (T)(1.0 / d.tau_max); p.omega_hi = (T)(1.0 / d.tau_min); }
+    p.omega_state = (T*)h->omega_state;
+    p.m_rho = (T*)h->macr[0]; p.m_ux = (T*)h->macr[1]; p.m_uy = (T*)h->macr[2];
+    p.m_uz = (T*)h->macr[3]; p.m_omega = (T*)h->macr[4];
+    p.zouhe_textbook = d.zouhe_textbook;
+    p.duct_R = (T)d.duct_outer_radius; p.duct_r = (T)d.duct_inner_radius;
+    p.w_out = (T)d.outer_rotation; p.u_out = (T)d.outer_velocity;
+    p.w_in = (T)d.inner_rotation; p.u_in = (T)d.inner_velocity;
+    p.tile_flag = h->tile_flag;
+    p.ibm_fx = (const T*)h->ibm_force[0]; p.ibm_fy = (const T*)h->ibm_force[1]; p.ibm_fz = (const T*)h->ibm_force[2];
+    p.tiles_x = h->tiles_x; p.tiles_y = h->tiles_y;
+}
+
+// Runs one kernel with parameters of the handle's compute type.
+template <typename F>
+inline int with_params(vlbm_ctx* h, F&& fn) {
+    if (h->tbytes == 8) {
+        Params<double> p;
+        fill_params(h, p);
+        return fn((void*)&p, p);
+    }
+    Params<float> p;
+    fill_params(h, p);
+    return fn((void*)&p, p);
+}
+
+inline int check_launch(int rc, const char* what) {
+    if (rc != 0) {
+        hipError_t e = hipGetLastError();
+        return fail(VLBM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    }
+    return VLBM_OK;
+}
+
+// immersed boundary stage (ibm_capi.cpp)
+void ibm_release(vlbm_ctx* h);
+int ibm_stage(vlbm_ctx* h);        // one stage on the current state
+int ibm_fill_macr(vlbm_ctx* h);    // macroscopics outside the flagged tiles
+int ibm_reset_tiles(vlbm_ctx* h);  // no flagged tiles: every node takes the constant force
+
+inline void release(vlbm_ctx* h) {
+    if (!h) return;
+    ibm_release(h);
+    for (void* p : h->lat) if (p) (void)hipFree(p);
+    for (void* p : h->macr) if (p) (void)hipFree(p);
+    for (void* p : h->uold) if (p) (void)hipFree(p);
+    if (h->omega_state) (void)hipFree(h->omega_state);
+    if (h->d_mask) (void)hipFree(h->d_mask);
+    if (h->partials) (void)hipFree(h->partials);
+    if (h->out3) (void)hipFree(h->out3);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+inline int dalloc(vlbm_ctx* h, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(VLBM_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    }
+    h->device_bytes += bytes;
+    return VLBM_OK;
+}
+
+// Host copy of a macroscopic field (compute type) as doubles.
+inline int fetch_field(vlbm_ctx* h, const void* dev, double* out) {
+    if (h->tbytes == 8) {
+        HIP_TRY(hipMemcpyAsync(out, dev, h->n * 8, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return VLBM_OK;
+    }
+    std::vector<float> tmp(h->n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), dev, h->n * 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (size_t k = 0; k < h->n; k++) out[k] = tmp[k];
+    return VLBM_OK;
+}
+
+inline int upload_field(vlbm_ctx* h, void* dev, const double* in) {
+    if (h->tbytes == 8) {
+        HIP_TRY(hipMemcpyAsync(dev, in, h->n * 8, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return VLBM_OK;
+    }
+    std::vector<float> tmp(h->n);
+    for (size_t k = 0; k < h->n; k++) tmp[k] = (float)in[k];
+    HIP_TRY(hipMemcpyAsync(dev, tmp.data(), h->n * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return VLBM_OK;
+}
+
+inline double pop_weight(int q, int i) { return q == 19 ? weight<19>(i) : weight<27>(i); }
+}  // namespace detail
+}  // namespace vlbm

@@ -1,0 +1,96 @@
+// Immersed boundary stage for rigid spheres: device data layout shared by the
+// kernels (ibm_kernels.h) and the host side (capi.cpp).
+//
+// Replaces src/CUDA/IBM/ibm.cu:6-1197, src/CUDA/IBM/collision/ibmCollision.cu
+// and the particle structs of src/CUDA/IBM/structs/. All particle arithmetic
+// is FP64; the fluid-side fields (macroscopics, force field, spread force) are
+// in the lattice's compute type.
+#pragma once
+#include <cstdint>
+
+namespace vlbm {
+
+constexpr int kMaxColl = 13;        // history slots: 7 walls + 6 partners (MAX_ACTIVE_COLLISIONS)
+constexpr int kFirstPartnerSlot = 7;
+constexpr int kMaxContacts = 16;    // sphere-sphere contacts recorded per sphere and step
+constexpr int kIbmBlock = 256;
+constexpr int kMaxSortKeys = 4096;  // spheres handled by the one-block cell sort
+
+// One sphere. The first 48 doubles are the C-ABI particle record
+// (vlbm_ibm_get_particles) in this order.
+struct IbmSphere {
+    double pos[3], vel[3], w[3], f[3], M[3], I[6];
+    double pos_old[3], vel_old[3], w_old[3], w_avg[3], w_pos[3], f_old[3], M_old[3], dP[3], dL[3];
+    double radius, volume, density;
+    int movable, first_node, n_nodes, cell;
+    int partner[kMaxColl];
+    int last[kMaxColl];
+    double tang[kMaxColl][3];
+};
+
+// Sphere-sphere contact of sphere i (the lower index) with sphere j.
+struct IbmContact {
+    int j, pad;
+    double fd[3];   // force on j; -fd acts on i
+    double mi[3];   // torque on i
+    double mj[3];   // torque on j
+};
+
+// Error bits raised on the device (read back by vlbm_ibm_status).
+enum IbmError : int {
+    kIbmErrContacts = 1,     // more than kMaxContacts contacts of one sphere
+    kIbmErrPartners = 2,     // more than kMaxContacts partners of one sphere found as the higher index
+};
+
+struct IbmParams {
+    int nx, ny, nz;
+    int periodic[3];
+    int tiles[3];
+    uint32_t n_tiles;
+    double thickness, alpha, fluid_density, g[3];
+    double pp_kn, pw_kn, pp_kt, pw_kt;     // Hertz / Mindlin constants
+    double pp_damp, pw_damp;               // -2 ln e / sqrt(pi^2 + ln^2 e)
+    double pp_mu, pw_mu;                   // friction coefficients
+    int ref_float;                         // float sqrt / rsqrt / fabs helpers as the reference
+    int n_spheres, n_nodes;
+    int step;                              // label of the stage (collision history)
+    IbmSphere* sph;
+    double *px, *py, *pz, *pS;             // Lagrangian nodes: position, area
+    double *pfx, *pfy, *pfz;               // accumulated force of the stage
+    double *pdx, *pdy, *pdz;               // force of the last iteration
+    double *pmx, *pmy, *pmz;               // its moment about the sphere centre
+    int* pid;                              // sphere of each node
+    uint8_t* tile_flag;
+    uint32_t* tile_list;
+    uint32_t* tile_count;
+    int cells[3];
+    double cell_w[3];
+    uint64_t* cell_keys;                   // (cell << 32) | sphere, sorted ascending
+    IbmContact* contacts;                  // [n_spheres][kMaxContacts]
+    int* n_contacts;
+    int* error;
+};
+
+template <typename T>
+struct IbmFluid {
+    T *rho, *ux, *uy, *uz;     // macroscopics the stage interpolates
+    T *fx, *fy, *fz;           // force field read by the next step in flagged tiles
+    T *ax, *ay, *az;           // force spread in the current iteration
+    T f0x, f0y, f0z;           // constant force
+};
+
+// Launchers (kernels_ibm.hip and the per-precision kernel files).
+int launch_ibm_macr(int q, int prec, bool mask_geom, const void* params, const void* fluid,
+                    const uint32_t* tile_list, const uint32_t* tile_count, uint32_t max_tiles, void* stream);
+int launch_ibm_stage(const IbmParams& P, int tbytes, const void* fluid, int stage_part, void* stream);
+
+// Parts of one stage, enqueued in this order (immersedBoundaryMethod, ibm.cu:6-192).
+enum IbmStagePart : int {
+    kIbmBegin = 0,        // old values, cell index, node force reset, tile flags
+    kIbmTiles = 1,        // compaction of the flagged tiles
+    kIbmContacts = 2,     // cell sort, contacts, force and torque sums, velocity update
+    kIbmIteration = 3,    // interpolation / spreading, sphere sums, velocity update, Euler sum
+    kIbmMove = 4,         // sphere and node movement
+};
+
+}  // namespace vlbm

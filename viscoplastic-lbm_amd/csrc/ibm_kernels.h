@@ -1,0 +1,744 @@
+// HIP kernels of the immersed boundary stage (rigid spheres), gfx950.
+//
+// One stage per time step, after the LBM step (src/CUDA/main.cu:369-389):
+//   begin     old values, cell index of each sphere, node forces reset, tiles
+//             touched by the 4^3 interpolation stencils flagged
+//   tiles     flagged 8^3 tiles compacted into a list (one block, fixed order)
+//   macr      rho, u of the post-step populations with the constant force on
+//             the flagged tiles only; force field reset to the constant there
+//   contacts  uniform cell list (one-block bitonic sort of (cell, sphere)),
+//             Hertz-Mindlin sphere-sphere contacts by the lower-index sphere,
+//             force / torque sums per sphere in a fixed order, walls, velocity
+//   iteration (x max_iterations) interpolation + force of each node, spread
+//             with atomics into the tile force buffer, per-sphere FP64 sums of
+//             the node forces / moments (fixed-order tree), velocity update,
+//             Euler sum u += F/(2 rho), force field += F on the flagged tiles
+//   move      sphere and node movement
+// Everything outside the flagged tiles is left alone: the reference's full-
+// lattice passes (gpuUpdateMacrIBM, gpuEulerSumIBMAuxsReset over the bordered
+// lattice) become passes over the tiles the spheres touch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "ibm.h"
+#include "lattice.h"
+
+namespace vlbm {
+
+// Peskin's 4-point kernel (src/CUDA/IBM/ibmGlobalFunctions.h:28-48).
+__device__ __forceinline__ double peskin4(double x) {
+    const double a = fabs(x);
+    if (a <= 1) return (1.0 / 8.0) * (3.0 - 2.0 * a + sqrt(1 + 4 * a - 4 * a * a));
+    if (a > 1.0 && a <= 2.0) return (1.0 / 8.0) * (5.0 - 2.0 * a - sqrt(-7.0 + 12.0 * a - 4.0 * a * a));
+    return 0.0;
+}
+
+// vector_length / vector_normalize (src/CUDA/globalFunctions.cu:81-100): the
+// reference evaluates them with sqrtf / rsqrtf also in FP64 builds.
+__device__ __forceinline__ double vec_len(const IbmParams& P, const double v[3]) {
+    const double s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (P.ref_float) return (double)sqrtf((float)s);
+    return sqrt(s);
+}
+__device__ __forceinline__ void vec_norm(const IbmParams& P, const double v[3], double o[3]) {
+    const double s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double inv = P.ref_float ? (double)(1.0f / sqrtf((float)s)) : 1.0 / sqrt(s);
+    if (isnan(inv) || isinf(inv)) { o[0] = o[1] = o[2] = 0; return; }
+    o[0] = v[0] * inv; o[1] = v[1] * inv; o[2] = v[2] * inv;
+}
+
+__device__ __forceinline__ int axis_n(const IbmParams& P, int a) { return a == 0 ? P.nx : (a == 1 ? P.ny : P.nz); }
+
+__device__ __forceinline__ double wrap_pos(const IbmParams& P, int a, double v) {
+    const double n = (double)axis_n(P, a);
+    return fmod(v + n, n);
+}
+
+// ---------------------------------------------------------------------------
+// stencil of a Lagrangian node (gpuForceInterpolationSpread, ibm.cu:194-330):
+// base int(x) - 1, four nodes per axis, clipped at walls, wrapped if periodic.
+// ---------------------------------------------------------------------------
+struct Stencil {
+    int base[3], mn[3], mx[3];
+    bool valid;
+};
+
+__device__ __forceinline__ Stencil make_stencil(const IbmParams& P, const double pos[3]) {
+    Stencil s;
+    for (int a = 0; a < 3; a++) {
+        const double pa = (pos[a] > -8.0 && pos[a] < (double)axis_n(P, a) + 8.0) ? pos[a] : -8.0;
+        s.base[a] = (int)pa - 2 + 1;
+        const int n = axis_n(P, a);
+        if (P.periodic[a]) {
+            s.mn[a] = 0;
+            s.mx[a] = 3;
+        } else {
+            s.mx[a] = ((s.base[a] + 3) < n) ? 3 : (n - 1 - s.base[a]);
+            s.mn[a] = (s.base[a] >= 0) ? 0 : -s.base[a];
+        }
+    }
+    s.valid = !(s.mx[0] <= 0 || s.mx[1] <= 0 || s.mx[2] <= 0 || s.mn[0] >= 4 || s.mn[1] >= 4 || s.mn[2] >= 4);
+    // nodes far outside the lattice (or not finite) touch nothing; this also
+    // keeps the integer stencil arithmetic in range
+    for (int a = 0; a < 3; a++)
+        if (!(pos[a] > -4.0 && pos[a] < (double)axis_n(P, a) + 4.0)) s.valid = false;
+    return s;
+}
+
+__device__ __forceinline__ int stencil_coord(const IbmParams& P, const Stencil& s, int a, int j) {
+    int c = s.base[a] + j;
+    if (P.periodic[a]) {
+        const int n = axis_n(P, a);
+        c = ((c % n) + n) % n;
+    }
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// begin: old values (gpuUpdateParticleOldValues, ibm.cu:995-1042), cell index,
+// tile flags cleared
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_begin(IbmParams P) {
+    const uint32_t gid = blockIdx.x * kIbmBlock + threadIdx.x;
+    const uint32_t stride = gridDim.x * kIbmBlock;
+    for (uint32_t t = gid; t < P.n_tiles; t += stride) P.tile_flag[t] = 0;
+    if (gid >= (uint32_t)P.n_spheres) return;
+    IbmSphere& c = P.sph[gid];
+    int cc[3];
+    for (int a = 0; a < 3; a++) {
+        c.dP[a] = 0; c.dL[a] = 0;
+        c.pos_old[a] = c.pos[a]; c.vel_old[a] = c.vel[a]; c.w_old[a] = c.w[a];
+        c.f_old[a] = c.f[a]; c.f[a] = 0; c.M[a] = 0;
+        int k = (int)floor(c.pos[a] / P.cell_w[a]);
+        k = k < 0 ? 0 : (k >= P.cells[a] ? P.cells[a] - 1 : k);
+        cc[a] = k;
+    }
+    c.cell = cc[0] + P.cells[0] * (cc[1] + P.cells[1] * cc[2]);
+}
+
+// node forces reset (gpuResetNodesForces, ibm.cu:650-664) and tiles flagged
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_mark(IbmParams P) {
+    const uint32_t i = blockIdx.x * kIbmBlock + threadIdx.x;
+    if (i >= (uint32_t)P.n_nodes) return;
+    P.pfx[i] = 0; P.pfy[i] = 0; P.pfz[i] = 0;
+    P.pdx[i] = 0; P.pdy[i] = 0; P.pdz[i] = 0;
+    P.pmx[i] = 0; P.pmy[i] = 0; P.pmz[i] = 0;
+    const double pos[3] = {P.px[i], P.py[i], P.pz[i]};
+    const Stencil s = make_stencil(P, pos);
+    if (!s.valid) return;
+    int tl[3][4], nt[3];
+    for (int a = 0; a < 3; a++) {
+        nt[a] = 0;
+        for (int j = s.mn[a]; j <= s.mx[a]; j++) {
+            const int t = stencil_coord(P, s, a, j) >> kTileShift;
+            bool seen = false;
+            for (int k = 0; k < nt[a]; k++) seen |= (tl[a][k] == t);
+            if (!seen) tl[a][nt[a]++] = t;
+        }
+    }
+    for (int c = 0; c < nt[2]; c++)
+        for (int b = 0; b < nt[1]; b++)
+            for (int a = 0; a < nt[0]; a++)
+                P.tile_flag[tl[0][a] + P.tiles[0] * (tl[1][b] + P.tiles[1] * tl[2][c])] = 1;
+}
+
+// flagged tiles -> list, ascending tile index (one block)
+__global__ __launch_bounds__(1024) void k_ibm_compact(IbmParams P) {
+    __shared__ uint32_t sums[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (P.n_tiles + 1023) / 1024;
+    const uint32_t beg = tid * per;
+    const uint32_t end = beg + per < P.n_tiles ? beg + per : P.n_tiles;
+    uint32_t cnt = 0;
+    for (uint32_t t = beg; t < end; t++) cnt += P.tile_flag[t] ? 1u : 0u;
+    sums[tid] = cnt;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = tid >= off ? sums[tid - off] : 0u;
+        __syncthreads();
+        sums[tid] += v;
+        __syncthreads();
+    }
+    uint32_t at = sums[tid] - cnt;
+    for (uint32_t t = beg; t < end; t++)
+        if (P.tile_flag[t]) P.tile_list[at++] = t;
+    if (tid == 1023) *P.tile_count = sums[1023];
+}
+
+// ---------------------------------------------------------------------------
+// cell list
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_ibm_sort_cells(IbmParams P) {
+    __shared__ uint64_t key[kMaxSortKeys];
+    const int n = P.n_spheres;
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = threadIdx.x; i < m; i += 1024)
+        key[i] = i < n ? (((uint64_t)(uint32_t)P.sph[i].cell << 32) | (uint32_t)i) : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += 1024) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & k) == 0;
+                    const uint64_t a = key[i], b = key[ixj];
+                    if ((a > b) == up) { key[i] = b; key[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < n; i += 1024) P.cell_keys[i] = key[i];
+}
+
+// cells whose spheres can touch a sphere of the given cell (distinct cells)
+__device__ __forceinline__ int neighbour_cells(const IbmParams& P, int cell, int out[27]) {
+    int cc[3];
+    cc[0] = cell % P.cells[0];
+    cc[1] = (cell / P.cells[0]) % P.cells[1];
+    cc[2] = cell / (P.cells[0] * P.cells[1]);
+    int v[3][3], nv[3];
+    for (int a = 0; a < 3; a++) {
+        const int nc = P.cells[a];
+        nv[a] = 0;
+        if (P.periodic[a] && nc <= 3) {
+            for (int k = 0; k < nc; k++) v[a][nv[a]++] = k;
+        } else {
+            for (int d = -1; d <= 1; d++) {
+                int k = cc[a] + d;
+                if (P.periodic[a]) k = (k + nc) % nc;
+                else if (k < 0 || k >= nc) continue;
+                v[a][nv[a]++] = k;
+            }
+        }
+    }
+    int n = 0;
+    for (int c = 0; c < nv[2]; c++)
+        for (int b = 0; b < nv[1]; b++)
+            for (int a = 0; a < nv[0]; a++) out[n++] = v[0][a] + P.cells[0] * (v[1][b] + P.cells[1] * v[2][c]);
+    return n;
+}
+
+__device__ __forceinline__ int lower_bound_key(const IbmParams& P, uint64_t k) {
+    int lo = 0, hi = P.n_spheres;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (P.cell_keys[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// contacts (src/CUDA/IBM/collision/ibmCollision.cu)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wall_slot(const double n[3]) {   // calculateWallIndex
+    return 7 + (1 - (int)n[0]) - 2 * (1 - (int)n[1]) - 3 * (1 - (int)n[2]);
+}
+__device__ __forceinline__ void end_collision(IbmSphere& c, int idx, int step) {
+    if (idx >= 0 && idx <= 6) {
+        c.last[idx] = -1;
+    } else if (idx >= kFirstPartnerSlot && idx < kMaxColl) {
+        if (step - c.last[idx] > 1) {
+            c.tang[idx][0] = c.tang[idx][1] = c.tang[idx][2] = 0;
+            c.last[idx] = -1;
+            c.partner[idx] = -1;
+        }
+    }
+}
+__device__ __forceinline__ int start_collision(IbmSphere& c, int partner, bool wall, const double n[3], int step) {
+    if (wall) {
+        const int idx = wall_slot(n);
+        c.tang[idx][0] = c.tang[idx][1] = c.tang[idx][2] = 0;
+        c.last[idx] = step;
+        return idx;
+    }
+    for (int i = kFirstPartnerSlot; i < kMaxColl; i++)
+        if (c.last[i] == -1) {
+            c.partner[i] = partner;
+            c.tang[i][0] = c.tang[i][1] = c.tang[i][2] = 0;
+            c.last[i] = step;
+            return i;
+        }
+    return -1;
+}
+__device__ __forceinline__ void update_tang(IbmSphere& c, int idx, const double d[3], int step, double out[3]) {
+    for (int a = 0; a < 3; a++) { c.tang[idx][a] += d[a]; out[a] = c.tang[idx][a]; }
+    c.last[idx] = step;
+}
+__device__ __forceinline__ int slot_of_partner(const IbmSphere& c, int partner, int step) {
+    for (int i = kFirstPartnerSlot; i < kMaxColl; i++)
+        if (c.partner[i] == partner && step - c.last[i] <= 1) return i;
+    return -1;
+}
+
+// sphereSphereGap (ibmCollision.cu:1061-1088)
+__device__ __forceinline__ double sphere_gap(const IbmParams& P, const IbmSphere& a, const IbmSphere& b) {
+    double dl[3];
+    for (int k = 0; k < 3; k++) {
+        dl[k] = a.pos[k] - b.pos[k];
+        if (P.periodic[k]) {
+            const double L = axis_n(P, k);
+            if (dl[k] > L / 2.0) dl[k] -= L;
+            if (dl[k] < -L / 2.0) dl[k] += L;
+        }
+    }
+    const double s2 = dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2];
+    const double dist = P.ref_float ? (double)sqrtf((float)s2) : sqrt(s2);
+    return dist - (a.radius + b.radius);
+}
+
+// sphereSphereCollision (ibmCollision.cu:1719-1913) seen from sphere i (the
+// lower index, whose history is updated). Returns false without contact.
+__device__ bool pair_contact(const IbmParams& P, IbmSphere& ci, const IbmSphere& cj, int row, int step,
+                             IbmContact& out) {
+    const double r_i = ci.radius, r_j = cj.radius;
+    const double m_i = ci.volume * ci.density, m_j = cj.volume * cj.density;
+    double dp[3];
+    for (int a = 0; a < 3; a++) {
+        double d = ci.pos[a] - cj.pos[a];
+        if (P.periodic[a]) {
+            const double L = axis_n(P, a);
+            if (fabs(ci.pos[a] - cj.pos[a]) > L / 2.0) d = ci.pos[a] < cj.pos[a] ? (ci.pos[a] + L - cj.pos[a]) : (ci.pos[a] - L - cj.pos[a]);
+        }
+        dp[a] = d;
+    }
+    const double md = sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]);
+    if (md > r_i + r_j) return false;
+    const double n[3] = {dp[0] / md, dp[1] / md, dp[2] / md};
+    const double disp = r_i + r_j - md;
+    const double G[3] = {ci.vel[0] - cj.vel[0], ci.vel[1] - cj.vel[1], ci.vel[2] - cj.vel[2]};
+    const double er = 1.0 / ((r_i + r_j) / (r_i * r_j));
+    const double em = 1.0 / ((m_i + m_j) / (m_i * m_j));
+    const double kn = P.pp_kn * sqrt(er);
+    const double kt = P.pp_kt * sqrt(er) * sqrt(fabs(disp));
+    const double dn = P.pp_damp * sqrt(em * kn), dt = P.pp_damp * sqrt(em * kt);
+    const double f_kn = -kn * sqrt(fabs(disp * disp * disp));
+    const double Gn = G[0] * n[0] + G[1] * n[1] + G[2] * n[2];
+    const double p25 = pow(fabs(disp), 0.25);
+    double fn[3];
+    for (int a = 0; a < 3; a++) fn[a] = f_kn * n[a] - dn * Gn * n[a] * p25;
+    const double f_n = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+    const double* wi = ci.w;
+    const double* wj = cj.w;
+    double Gct[3];
+    Gct[0] = G[0] + r_i * (wi[1] * n[2] - wi[2] * n[1]) + r_j * (wj[1] * n[2] - wj[2] * n[1]) - Gn * n[0];
+    Gct[1] = G[1] + r_i * (wi[2] * n[0] - wi[0] * n[2]) + r_j * (wj[2] * n[0] - wj[0] * n[2]) - Gn * n[1];
+    Gct[2] = G[2] + r_i * (wi[0] * n[1] - wi[1] * n[0]) + r_j * (wj[0] * n[1] - wj[1] * n[0]) - Gn * n[2];
+    double mag = sqrt(Gct[0] * Gct[0] + Gct[1] * Gct[1] + Gct[2] * Gct[2]);
+    double t[3] = {0, 0, 0};
+    if (mag != 0)
+        for (int a = 0; a < 3; a++) t[a] = Gct[a] / mag;
+    double td[3];
+    int ti = slot_of_partner(ci, row, step);
+    if (ti == -1) {
+        ti = start_collision(ci, row, false, n, step);
+        for (int a = 0; a < 3; a++) td[a] = Gct[a];
+    } else if (step - ci.last[ti] > 1) {
+        end_collision(ci, ti, step);
+        ti = start_collision(ci, row, false, n, step);
+        for (int a = 0; a < 3; a++) td[a] = Gct[a];
+    } else {
+        update_tang(ci, ti, G, step, td);   // the reference accumulates G here
+    }
+    double ft[3];
+    for (int a = 0; a < 3; a++) ft[a] = -kt * td[a] - dt * Gct[a] * pow(fabs(td[a]), 0.25);
+    mag = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+    if (mag > P.pp_mu * fabs(f_n)) {
+        const double mg[3] = {-Gct[0], -Gct[1], -Gct[2]};
+        if (ti >= 0) update_tang(ci, ti, mg, step, td);   // no free slot: the reference writes slot -1
+        for (int a = 0; a < 3; a++) ft[a] = -P.pp_mu * f_n * t[a];
+    }
+    out.j = row;
+    out.pad = 0;
+    for (int a = 0; a < 3; a++) out.fd[a] = fn[a] + ft[a];
+    out.mi[0] = r_i * (n[1] * ft[2] - n[2] * ft[1]);
+    out.mi[1] = r_i * (n[2] * ft[0] - n[0] * ft[2]);
+    out.mi[2] = r_i * (n[0] * ft[1] - n[1] * ft[0]);
+    out.mj[0] = r_j * (n[1] * ft[2] - n[2] * ft[1]);
+    out.mj[1] = r_j * (n[2] * ft[0] - n[0] * ft[2]);
+    out.mj[2] = r_j * (n[0] * ft[1] - n[1] * ft[0]);
+    return true;
+}
+
+// sphereWallCollision (ibmCollision.cu:394-508), accumulating into f / M
+__device__ void wall_contact(const IbmParams& P, IbmSphere& c, const double wall_normal[3], double disp, int step,
+                             double f[3], double M[3]) {
+    const double m_i = c.volume * c.density, r_i = c.radius;
+    const double n[3] = {-wall_normal[0], -wall_normal[1], -wall_normal[2]};
+    const double* G = c.vel;
+    const double* w_i = c.w;
+    const double kn = P.pw_kn * sqrt(fabs(r_i));
+    const double kt = P.pw_kt * sqrt(r_i) * sqrt(fabs(disp));
+    const double dn = P.pw_damp * sqrt(m_i * kn), dt = P.pw_damp * sqrt(m_i * kt);
+    const double f_kn = -kn * sqrt(fabs(disp * disp * disp));
+    const double Gn = G[0] * n[0] + G[1] * n[1] + G[2] * n[2];
+    const double p25 = pow(fabs(disp), 0.25);
+    double fn[3];
+    for (int a = 0; a < 3; a++) fn[a] = f_kn * n[a] - dn * Gn * n[a] * p25;
+    const double f_n = sqrt(fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2]);
+    double Gct[3];
+    Gct[0] = G[0] + r_i * (w_i[1] * n[2] - w_i[2] * n[1]) - Gn * n[0];
+    Gct[1] = G[1] + r_i * (w_i[2] * n[0] - w_i[0] * n[2]) - Gn * n[1];
+    Gct[2] = G[2] + r_i * (w_i[0] * n[1] - w_i[1] * n[0]) - Gn * n[2];
+    double mag = sqrt(Gct[0] * Gct[0] + Gct[1] * Gct[1] + Gct[2] * Gct[2]);
+    double t[3] = {0, 0, 0};
+    if (mag != 0)
+        for (int a = 0; a < 3; a++) t[a] = Gct[a] / mag;
+    const int ti = wall_slot(n);
+    double td[3];
+    if (step - c.last[ti] > 1) {
+        end_collision(c, ti, step);
+        start_collision(c, ti, true, n, step);
+        for (int a = 0; a < 3; a++) td[a] = Gct[a];
+    } else {
+        update_tang(c, ti, Gct, step, td);
+    }
+    double ft[3];
+    for (int a = 0; a < 3; a++) ft[a] = -kt * td[a] - dt * Gct[a] * pow(fabs(td[a]), 0.25);
+    mag = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+    const double fnab = P.ref_float ? (double)fabsf((float)f_n) : fabs(f_n);
+    if (mag > P.pw_mu * fnab) {
+        const double mg[3] = {-Gct[0], -Gct[1], -Gct[2]};
+        update_tang(c, ti, mg, step, td);
+        for (int a = 0; a < 3; a++) ft[a] = -P.pw_mu * f_n * t[a];
+    }
+    for (int a = 0; a < 3; a++) f[a] += fn[a] + ft[a];
+    M[0] += r_i * (n[1] * ft[2] - n[2] * ft[1]);
+    M[1] += r_i * (n[2] * ft[0] - n[0] * ft[2]);
+    M[2] += r_i * (n[0] * ft[1] - n[1] * ft[0]);
+}
+
+// sphere-sphere contacts of each sphere with higher-index spheres of the
+// neighbouring cells, in ascending partner order (the reference's pair order
+// for that sphere's history)
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_contacts(IbmParams P) {
+    const int i = blockIdx.x * kIbmBlock + threadIdx.x;
+    if (i >= P.n_spheres) return;
+    IbmSphere& ci = P.sph[i];
+    int cand[kMaxContacts];
+    int nc = 0;
+    int cells[27];
+    const int ncell = neighbour_cells(P, ci.cell, cells);
+    for (int c = 0; c < ncell; c++) {
+        const uint64_t k0 = (uint64_t)(uint32_t)cells[c] << 32;
+        for (int k = lower_bound_key(P, k0); k < P.n_spheres && (P.cell_keys[k] >> 32) == (uint64_t)(uint32_t)cells[c]; k++) {
+            const int j = (int)(P.cell_keys[k] & 0xffffffffu);
+            if (j <= i) continue;
+            const IbmSphere& cj = P.sph[j];
+            if (!ci.movable && !cj.movable) continue;
+            if (!(sphere_gap(P, ci, cj) < 0)) continue;
+            if (nc == kMaxContacts) { atomicOr(P.error, (int)kIbmErrContacts); continue; }
+            int at = nc++;
+            while (at > 0 && cand[at - 1] > j) { cand[at] = cand[at - 1]; at--; }
+            cand[at] = j;
+        }
+    }
+    int rec = 0;
+    for (int k = 0; k < nc; k++) {
+        IbmContact ct;
+        if (pair_contact(P, ci, P.sph[cand[k]], cand[k], P.step, ct)) P.contacts[i * kMaxContacts + rec++] = ct;
+    }
+    P.n_contacts[i] = rec;
+}
+
+// checkCollisionWallsSphere (ibmCollision.cu:188-251): walls at 0 and N-1
+__device__ void wall_contacts(const IbmParams& P, IbmSphere& c, double f[3], double M[3]) {
+    for (int a = 0; a < 3; a++) {
+        if (P.periodic[a]) continue;
+        double nrm[3] = {0, 0, 0};
+        nrm[a] = 1;
+        double dist = c.pos[a] * 1.0 - 0.0;
+        if (dist < c.radius) wall_contact(P, c, nrm, c.radius - dist, P.step, f, M);
+        nrm[a] = -1;
+        dist = (double)(axis_n(P, a) - 1) + (-c.pos[a]);
+        if (dist < c.radius) wall_contact(P, c, nrm, c.radius - dist, P.step, f, M);
+    }
+}
+
+// force / torque of the contacts in the reference's accumulation order for
+// each sphere: as the higher index (ascending partner), as the lower index
+// (ascending partner), then the walls
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_contact_sum(IbmParams P) {
+    const int p = blockIdx.x * kIbmBlock + threadIdx.x;
+    if (p >= P.n_spheres) return;
+    IbmSphere& cp = P.sph[p];
+    double f[3] = {cp.f[0], cp.f[1], cp.f[2]};
+    double M[3] = {cp.M[0], cp.M[1], cp.M[2]};
+    int from_i[kMaxContacts], from_k[kMaxContacts];
+    int nr = 0;
+    int cells[27];
+    const int ncell = neighbour_cells(P, cp.cell, cells);
+    for (int c = 0; c < ncell; c++) {
+        const uint64_t k0 = (uint64_t)(uint32_t)cells[c] << 32;
+        for (int k = lower_bound_key(P, k0); k < P.n_spheres && (P.cell_keys[k] >> 32) == (uint64_t)(uint32_t)cells[c]; k++) {
+            const int i = (int)(P.cell_keys[k] & 0xffffffffu);
+            if (i >= p) continue;
+            const int nci = P.n_contacts[i];
+            for (int r = 0; r < nci; r++) {
+                if (P.contacts[i * kMaxContacts + r].j != p) continue;
+                if (nr == kMaxContacts) { atomicOr(P.error, (int)kIbmErrPartners); continue; }
+                int at = nr++;
+                while (at > 0 && from_i[at - 1] > i) { from_i[at] = from_i[at - 1]; from_k[at] = from_k[at - 1]; at--; }
+                from_i[at] = i;
+                from_k[at] = r;
+            }
+        }
+    }
+    for (int r = 0; r < nr; r++) {
+        const IbmContact& ct = P.contacts[from_i[r] * kMaxContacts + from_k[r]];
+        for (int a = 0; a < 3; a++) { f[a] += ct.fd[a]; M[a] += ct.mj[a]; }
+    }
+    const int own = P.n_contacts[p];
+    for (int r = 0; r < own; r++) {
+        const IbmContact& ct = P.contacts[p * kMaxContacts + r];
+        for (int a = 0; a < 3; a++) { f[a] += -ct.fd[a]; M[a] += ct.mi[a]; }
+    }
+    if (cp.movable) wall_contacts(P, cp, f, M);
+    for (int a = 0; a < 3; a++) { cp.f[a] = f[a]; cp.M[a] = M[a]; }
+}
+
+// gpuUpdateParticleCenterVelocityAndRotation (ibm.cu:755-897)
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_velocity(IbmParams P) {
+    const int s = blockIdx.x * kIbmBlock + threadIdx.x;
+    if (s >= P.n_spheres) return;
+    IbmSphere& c = P.sph[s];
+    if (!c.movable) return;
+    const double al = P.alpha;
+    const double inv_volume = 1 / c.volume;
+    for (int a = 0; a < 3; a++)
+        c.vel[a] = c.vel_old[a] + (((c.f_old[a] * (1.0 - al) + c.f[a] * al) + c.dP[a]) * inv_volume +
+                                   (c.density - P.fluid_density) * P.g[a]) / (c.density);
+    const double xx = c.I[0], yy = c.I[1], zz = c.I[2], xy = c.I[3], xz = c.I[4], yz = c.I[5];
+    const double inv_det_neg = 1.0 / (zz * xy * xy + yy * xz * xz + xx * yz * yz - xx * yy * zz - 2 * xy * xz * yz);
+    double wA[3], LM[3];
+    for (int a = 0; a < 3; a++) {
+        wA[a] = (c.w_old[a] * (1.0 - al) + c.w_old[a] * al);
+        LM[a] = c.dL[a] + (c.M[a] * al + c.M_old[a] * (1.0 - al));
+    }
+    const double Iwx = xx * wA[0] + xy * wA[1] + xz * wA[2];
+    const double Iwy = xy * wA[0] + yy * wA[1] + yz * wA[2];
+    const double Iwz = xz * wA[0] + yz * wA[1] + zz * wA[2];
+    const double Ax = LM[0] + wA[2] * Iwy - wA[1] * Iwz;
+    const double Ay = LM[1] + wA[0] * Iwz - wA[2] * Iwx;
+    const double Az = LM[2] + wA[1] * Iwx - wA[0] * Iwy;
+    double wN[3];
+    wN[0] = c.w_old[0] + ((yz * yz - yy * zz) * Ax - (xy * yz - xz * yy) * Az - (xz * yz - xy * zz) * Ay) * inv_det_neg;
+    wN[1] = c.w_old[1] + ((xz * xz - xx * zz) * Ay - (xy * xz - xx * yz) * Az - (xz * yz - xy * zz) * Ax) * inv_det_neg;
+    wN[2] = c.w_old[2] + ((xy * xy - xx * yy) * Az - (xy * xz - xx * yz) * Ay - (xy * yz - xz * yy) * Ax) * inv_det_neg;
+    // inertia rotated by the average rotation vector: R I R^T
+    const double ang = vec_len(P, wA);
+    double ax[3];
+    vec_norm(P, wA, ax);
+    const double h = ang / 2.0;
+    const double qw = cos(h), qx = ax[0] * sin(h), qy = ax[1] * sin(h), qz = ax[2] * sin(h);
+    double R[3][3];
+    R[0][0] = 1 - 2 * (qy * qy + qz * qz); R[0][1] = 2 * (qx * qy - qw * qz); R[0][2] = 2 * (qx * qz + qw * qy);
+    R[1][0] = 2 * (qx * qy + qw * qz); R[1][1] = 1 - 2 * (qx * qx + qz * qz); R[1][2] = 2 * (qy * qz - qw * qx);
+    R[2][0] = 2 * (qx * qz - qw * qy); R[2][1] = 2 * (qy * qz + qw * qx); R[2][2] = 1 - 2 * (qx * qx + qy * qy);
+    const double M3[3][3] = {{xx, xy, xz}, {xy, yy, yz}, {xz, yz, zz}};
+    double Tm[3][3], N3[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0;
+            for (int k = 0; k < 3; k++) acc += R[i][k] * M3[k][j];
+            Tm[i][j] = acc;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double acc = 0;
+            for (int k = 0; k < 3; k++) acc += Tm[i][k] * R[j][k];
+            N3[i][j] = acc;
+        }
+    for (int a = 0; a < 3; a++) c.w[a] = wN[a];
+    c.I[0] = N3[0][0]; c.I[1] = N3[1][1]; c.I[2] = N3[2][2];
+    c.I[3] = N3[0][1]; c.I[4] = N3[0][2]; c.I[5] = N3[1][2];
+}
+
+// ---------------------------------------------------------------------------
+// interpolation / spreading of one iteration (ibm.cu:194-542)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_interp_spread(IbmParams P, IbmFluid<T> fl) {
+    const uint32_t i = blockIdx.x * kIbmBlock + threadIdx.x;
+    if (i >= (uint32_t)P.n_nodes) return;
+    const double pos[3] = {P.px[i], P.py[i], P.pz[i]};
+    const Stencil s = make_stencil(P, pos);
+    if (!s.valid) {
+        P.pdx[i] = 0; P.pdy[i] = 0; P.pdz[i] = 0;
+        P.pmx[i] = 0; P.pmy[i] = 0; P.pmz[i] = 0;
+        return;
+    }
+    double sv[3][4];
+    for (int a = 0; a < 3; a++)
+        for (int j = 0; j < 4; j++) sv[a][j] = (j >= s.mn[a] && j <= s.mx[a]) ? peskin4(s.base[a] + j - pos[a]) : 0.0;
+    int cx[4], cy[4], cz[4];
+    for (int j = 0; j < 4; j++) {
+        cx[j] = stencil_coord(P, s, 0, j);
+        cy[j] = stencil_coord(P, s, 1, j);
+        cz[j] = stencil_coord(P, s, 2, j);
+    }
+    double rv = 0, uxv = 0, uyv = 0, uzv = 0;
+    for (int zk = s.mn[2]; zk <= s.mx[2]; zk++)
+        for (int yj = s.mn[1]; yj <= s.mx[1]; yj++) {
+            const double a1 = sv[2][zk] * sv[1][yj];
+            const size_t row = ((size_t)cz[zk] * P.ny + cy[yj]) * P.nx;
+            for (int xi = s.mn[0]; xi <= s.mx[0]; xi++) {
+                const double a = a1 * sv[0][xi];
+                const size_t k = row + cx[xi];
+                rv += (double)fl.rho[k] * a;
+                uxv += (double)fl.ux[k] * a;
+                uyv += (double)fl.uy[k] * a;
+                uzv += (double)fl.uz[k] * a;
+            }
+        }
+    const IbmSphere& c = P.sph[P.pid[i]];
+    double d[3] = {pos[0] - c.pos[0], pos[1] - c.pos[1], pos[2] - c.pos[2]};
+    for (int a = 0; a < 3; a++) {
+        if (!P.periodic[a]) continue;
+        const double n = axis_n(P, a);
+        if (fabs(d[a]) > n / 2.0) d[a] = (d[a] < 0) ? (pos[a] + n) - c.pos[a] : (pos[a] - n) - c.pos[a];
+    }
+    double uc[3] = {0, 0, 0};
+    if (c.movable) {
+        uc[0] = c.vel[0] + (c.w[1] * d[2] - c.w[2] * d[1]);
+        uc[1] = c.vel[1] + (c.w[2] * d[0] - c.w[0] * d[2]);
+        uc[2] = c.vel[2] + (c.w[0] * d[1] - c.w[1] * d[0]);
+    }
+    const double aux = 2 * rv * P.pS[i] * P.thickness;
+    const double dF[3] = {aux * (uxv - uc[0]), aux * (uyv - uc[1]), aux * (uzv - uc[2])};
+    for (int zk = s.mn[2]; zk <= s.mx[2]; zk++)
+        for (int yj = s.mn[1]; yj <= s.mx[1]; yj++) {
+            const double a1 = sv[2][zk] * sv[1][yj];
+            const size_t row = ((size_t)cz[zk] * P.ny + cy[yj]) * P.nx;
+            for (int xi = s.mn[0]; xi <= s.mx[0]; xi++) {
+                const double a = a1 * sv[0][xi];
+                const size_t k = row + cx[xi];
+                unsafeAtomicAdd(fl.ax + k, (T)(-dF[0] * a));
+                unsafeAtomicAdd(fl.ay + k, (T)(-dF[1] * a));
+                unsafeAtomicAdd(fl.az + k, (T)(-dF[2] * a));
+            }
+        }
+    P.pfx[i] += dF[0]; P.pfy[i] += dF[1]; P.pfz[i] += dF[2];
+    P.pdx[i] = dF[0]; P.pdy[i] = dF[1]; P.pdz[i] = dF[2];
+    P.pmx[i] = d[1] * dF[2] - d[2] * dF[1];
+    P.pmy[i] = d[2] * dF[0] - d[0] * dF[2];
+    P.pmz[i] = d[0] * dF[1] - d[1] * dF[0];
+}
+
+// per-sphere FP64 sums of the node forces / moments (fixed-order tree)
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_reduce_nodes(IbmParams P) {
+    __shared__ double sh[6][kIbmBlock / 64];
+    IbmSphere& c = P.sph[blockIdx.x];
+    const int first = c.first_node, n = c.n_nodes;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = threadIdx.x; k < n; k += kIbmBlock) {
+        const int i = first + k;
+        acc[0] += P.pdx[i]; acc[1] += P.pdy[i]; acc[2] += P.pdz[i];
+        acc[3] += P.pmx[i]; acc[4] += P.pmy[i]; acc[5] += P.pmz[i];
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 6; a++) acc[a] += __shfl_down(acc[a], off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int a = 0; a < 6; a++) sh[a][wave] = acc[a];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[6] = {0, 0, 0, 0, 0, 0};
+        for (int w = 0; w < kIbmBlock / 64; w++)
+            for (int a = 0; a < 6; a++) t[a] += sh[a][w];
+        for (int a = 0; a < 3; a++) { c.f[a] += t[a]; c.M[a] += t[3 + a]; }
+    }
+}
+
+// gpuEulerSumIBMAuxsReset (particleEulerNodesUpdate.cu:287-330) on the
+// flagged tiles: u += F_spread / (2 rho), force field += F_spread, buffer cleared
+template <typename T>
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_euler_sum(IbmParams P, IbmFluid<T> fl) {
+    const uint32_t count = *P.tile_count;
+    for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
+        const uint32_t tile = P.tile_list[t];
+        const int tx = (int)(tile % (uint32_t)P.tiles[0]);
+        const int ty = (int)((tile / (uint32_t)P.tiles[0]) % (uint32_t)P.tiles[1]);
+        const int tz = (int)(tile / ((uint32_t)P.tiles[0] * (uint32_t)P.tiles[1]));
+        for (int l = threadIdx.x; l < kTileEdge * kTileEdge * kTileEdge; l += kIbmBlock) {
+            const int x = (tx << kTileShift) + (l & (kTileEdge - 1));
+            const int y = (ty << kTileShift) + ((l >> kTileShift) & (kTileEdge - 1));
+            const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
+            if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
+            const size_t k = ((size_t)z * P.ny + y) * P.nx + x;
+            const T ax = fl.ax[k], ay = fl.ay[k], az = fl.az[k];
+            const T h = (T)0.5 * ((T)1 / fl.rho[k]);
+            fl.ux[k] += ax * h; fl.uy[k] += ay * h; fl.uz[k] += az * h;
+            fl.fx[k] += ax; fl.fy[k] += ay; fl.fz[k] += az;
+            fl.ax[k] = 0; fl.ay[k] = 0; fl.az[k] = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// movement (gpuParticleMovement ibm.cu:899-993, gpuParticleNodeMovement
+// ibm.cu:1044-1197)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_move(IbmParams P) {
+    const int s = blockIdx.x * kIbmBlock + threadIdx.x;
+    if (s >= P.n_spheres) return;
+    IbmSphere& c = P.sph[s];
+    if (!c.movable) return;
+    const double al = P.alpha;
+    for (int a = 0; a < 3; a++) {
+        const double dp = (c.vel[a] * al + c.vel_old[a] * (1.0 - al));
+        c.pos[a] = P.periodic[a] ? wrap_pos(P, a, c.pos[a] + dp) : c.pos[a] + dp;
+    }
+    for (int a = 0; a < 3; a++) {
+        c.w_avg[a] = (c.w[a] * al + c.w_old[a] * (1.0 - al));
+        c.w_pos[a] += c.w_avg[a];
+    }
+}
+
+// the rotation keeps the reference's (qi*qj) - (q0*qj) entry (ibm.cu:1174)
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_move_nodes(IbmParams P) {
+    const uint32_t i = blockIdx.x * kIbmBlock + threadIdx.x;
+    if (i >= (uint32_t)P.n_nodes) return;
+    const IbmSphere& c = P.sph[P.pid[i]];
+    if (!c.movable) return;
+    double p3[3] = {P.px[i], P.py[i], P.pz[i]};
+    const double wn = sqrt(c.w_avg[0] * c.w_avg[0] + c.w_avg[1] * c.w_avg[1] + c.w_avg[2] * c.w_avg[2]);
+    if (wn <= 1e-8) {
+        for (int a = 0; a < 3; a++) {
+            double dd = c.pos[a] - c.pos_old[a];
+            if (P.periodic[a]) {
+                const double n = axis_n(P, a);
+                if (fabs(dd) > n / 2.0) dd = (c.pos[a] < c.pos_old[a]) ? (c.pos[a] + n) - c.pos_old[a] : (c.pos[a] - n) - c.pos_old[a];
+                p3[a] = wrap_pos(P, a, p3[a] + dd);
+            } else {
+                p3[a] += dd;
+            }
+        }
+    } else {
+        double v[3];
+        for (int a = 0; a < 3; a++) {
+            v[a] = p3[a] - c.pos_old[a];
+            if (P.periodic[a]) {
+                const double n = axis_n(P, a);
+                if (fabs(v[a]) > n / 2.0) {
+                    if (p3[a] < c.pos_old[a]) p3[a] += n;
+                    else p3[a] -= n;
+                }
+                v[a] = p3[a] - c.pos_old[a];
+            }
+        }
+        const double q0 = cos(0.5 * wn);
+        const double qi = (c.w_avg[0] / wn) * sin(0.5 * wn);
+        const double qj = (c.w_avg[1] / wn) * sin(0.5 * wn);
+        const double qk = (c.w_avg[2] / wn) * sin(0.5 * wn);
+        const double t = (q0 * q0) - 0.5;
+        const double np3[3] = {
+            c.pos[0] + 2 * ((t + (qi * qi)) * v[0] + ((qi * qj) - (q0 * qk)) * v[1] + ((qi * qk) + (q0 * qj)) * v[2]),
+            c.pos[1] + 2 * (((qi * qj) + (q0 * qk)) * v[0] + (t + (qj * qj)) * v[1] + ((qj * qk) - (q0 * qi)) * v[2]),
+            c.pos[2] + 2 * (((qi * qj) - (q0 * qj)) * v[0] + ((qj * qk) + (q0 * qi)) * v[1] + (t + (qk * qk)) * v[2])};
+        for (int a = 0; a < 3; a++) p3[a] = P.periodic[a] ? wrap_pos(P, a, np3[a]) : np3[a];
+    }
+    P.px[i] = p3[0]; P.py[i] = p3[1]; P.pz[i] = p3[2];
+}
+
+}  // namespace vlbm

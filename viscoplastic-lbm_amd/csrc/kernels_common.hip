@@ -1,11 +1,18 @@
 // Precision routing, initialisation, reduction and conversion kernels.
 #include "kernels_impl.h"
+#include "ibm.h"
 
 namespace vlbm {
 
-int launch_step_fp64(int q, int rheo, bool mask, bool direct, const void* p, void* stream);
-int launch_step_fp32(int q, int rheo, bool mask, bool direct, const void* p, void* stream);
-int launch_step_fp16(int q, int rheo, bool mask, bool direct, const void* p, void* stream);
+int launch_step_fp64(int q, int rheo, bool mask, bool direct, bool ibm, const void* p, void* stream);
+int launch_step_fp32(int q, int rheo, bool mask, bool direct, bool ibm, const void* p, void* stream);
+int launch_step_fp16(int q, int rheo, bool mask, bool direct, bool ibm, const void* p, void* stream);
+int launch_ibm_macr_fp64(int q, bool mask, const void* p, const uint32_t* list, const uint32_t* count,
+                         uint32_t max_tiles, void* stream);
+int launch_ibm_macr_fp32(int q, bool mask, const void* p, const uint32_t* list, const uint32_t* count,
+                         uint32_t max_tiles, void* stream);
+int launch_ibm_macr_fp16(int q, bool mask, const void* p, const uint32_t* list, const uint32_t* count,
+                         uint32_t max_tiles, void* stream);
 int launch_aux_fp64(int q, int rheo, bool mask, const void* p, void* stream);
 int launch_aux_fp32(int q, int rheo, bool mask, const void* p, void* stream);
 int launch_aux_fp16(int q, int rheo, bool mask, const void* p, void* stream);
@@ -26,10 +33,17 @@ __global__ void k_reduce_final(const double* partials, int nblocks, double* out3
 
 static inline dim3 grid_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
-int launch_step(int q, int prec, int rheo, bool mask, bool direct, const void* p, void* stream) {
-    if (prec == 8) return launch_step_fp64(q, rheo, mask, direct, p, stream);
-    if (prec == 4) return launch_step_fp32(q, rheo, mask, direct, p, stream);
-    return launch_step_fp16(q, rheo, mask, direct, p, stream);
+int launch_step(int q, int prec, int rheo, bool mask, bool direct, const void* p, void* stream, bool ibm) {
+    if (prec == 8) return launch_step_fp64(q, rheo, mask, direct, ibm, p, stream);
+    if (prec == 4) return launch_step_fp32(q, rheo, mask, direct, ibm, p, stream);
+    return launch_step_fp16(q, rheo, mask, direct, ibm, p, stream);
+}
+
+int launch_ibm_macr(int q, int prec, bool mask, const void* p, const void* /*fluid*/, const uint32_t* list,
+                    const uint32_t* count, uint32_t max_tiles, void* stream) {
+    if (prec == 8) return launch_ibm_macr_fp64(q, mask, p, list, count, max_tiles, stream);
+    if (prec == 4) return launch_ibm_macr_fp32(q, mask, p, list, count, max_tiles, stream);
+    return launch_ibm_macr_fp16(q, mask, p, list, count, max_tiles, stream);
 }
 
 int launch_aux(int q, int prec, int rheo, bool mask, const void* p, void* stream) {

@@ -1,0 +1,56 @@
+// Launch sequence of the immersed boundary stage (particles FP64; fluid-side
+// fields in the lattice's compute type). The tile macroscopics kernel, which
+// reads the populations, is launched from the per-precision kernel files.
+#include "ibm_kernels.h"
+
+namespace vlbm {
+
+static inline dim3 blocks_for(uint32_t n) { return dim3(n ? (n + kIbmBlock - 1) / kIbmBlock : 1u); }
+
+template <typename T>
+static int stage_part(const IbmParams& P, const IbmFluid<T>& fl, int part, hipStream_t st) {
+    const dim3 blk(kIbmBlock);
+    const uint32_t ns = (uint32_t)P.n_spheres, nn = (uint32_t)P.n_nodes;
+    // grid-stride passes over the tile list: enough blocks for the tiles the
+    // spheres can touch, at most 2048
+    const dim3 tile_grid(P.n_tiles < 2048u ? (P.n_tiles ? P.n_tiles : 1u) : 2048u);
+    switch (part) {
+    case kIbmBegin:
+        hipLaunchKernelGGL(k_ibm_begin, blocks_for(ns > P.n_tiles ? ns : P.n_tiles), blk, 0, st, P);
+        if (nn) hipLaunchKernelGGL(k_ibm_mark, blocks_for(nn), blk, 0, st, P);
+        break;
+    case kIbmTiles:
+        hipLaunchKernelGGL(k_ibm_compact, dim3(1), dim3(1024), 0, st, P);
+        break;
+    case kIbmContacts:
+        if (!ns) break;
+        hipLaunchKernelGGL(k_ibm_sort_cells, dim3(1), dim3(1024), 0, st, P);
+        hipLaunchKernelGGL(k_ibm_contacts, blocks_for(ns), blk, 0, st, P);
+        hipLaunchKernelGGL(k_ibm_contact_sum, blocks_for(ns), blk, 0, st, P);
+        hipLaunchKernelGGL(k_ibm_velocity, blocks_for(ns), blk, 0, st, P);
+        break;
+    case kIbmIteration:
+        if (nn) {
+            hipLaunchKernelGGL((k_ibm_interp_spread<T>), blocks_for(nn), blk, 0, st, P, fl);
+            hipLaunchKernelGGL(k_ibm_reduce_nodes, dim3(ns), blk, 0, st, P);
+        }
+        if (ns) hipLaunchKernelGGL(k_ibm_velocity, blocks_for(ns), blk, 0, st, P);
+        hipLaunchKernelGGL((k_ibm_euler_sum<T>), tile_grid, blk, 0, st, P, fl);
+        break;
+    case kIbmMove:
+        if (ns) hipLaunchKernelGGL(k_ibm_move, blocks_for(ns), blk, 0, st, P);
+        if (nn) hipLaunchKernelGGL(k_ibm_move_nodes, blocks_for(nn), blk, 0, st, P);
+        break;
+    default:
+        return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_ibm_stage(const IbmParams& P, int tbytes, const void* fluid, int part, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (tbytes == 8) return stage_part<double>(P, *(const IbmFluid<double>*)fluid, part, st);
+    return stage_part<float>(P, *(const IbmFluid<float>*)fluid, part, st);
+}
+
+}  // namespace vlbm

@@ -390,9 +390,27 @@ struct Moments {
     T dr, rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, omega;
 };
 
+// Body force acting on a node: the constant force, or the per-node field of
+// the immersed boundary stage.
+template <typename T>
+struct Force {
+    T x, y, z;
+};
+
+template <typename T>
+__device__ __forceinline__ Force<T> const_force(const Params<T>& p) { return Force<T>{p.fx, p.fy, p.fz}; }
+
+// Tile of the IBM stage holding the node.
+template <typename T>
+__device__ __forceinline__ uint32_t tile_of(const Params<T>& p, int x, int y, int z) {
+    return (uint32_t)(x >> kTileShift) +
+           (uint32_t)p.tiles_x * ((uint32_t)(y >> kTileShift) + (uint32_t)p.tiles_y * (uint32_t)(z >> kTileShift));
+}
+
 // omega_old: power-law state of the node (ignored otherwise).
 template <int Q, int RHEO, typename T>
-__device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<T>& p, T omega_old) {
+__device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<T>& p, T omega_old,
+                                                       const Force<T>& F) {
     Moments<Q, RHEO, T> m;
     T dr = 0, jx = 0, jy = 0, jz = 0, sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
 #pragma unroll
@@ -408,9 +426,9 @@ __device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<
     }
     const T rho = (T)1 + dr;
     const T inv = (T)1 / rho;
-    const T ux = (jx + (T)0.5 * p.fx) * inv;
-    const T uy = (jy + (T)0.5 * p.fy) * inv;
-    const T uz = (jz + (T)0.5 * p.fz) * inv;
+    const T ux = (jx + (T)0.5 * F.x) * inv;
+    const T uy = (jy + (T)0.5 * F.y) * inv;
+    const T uz = (jz + (T)0.5 * F.z) * inv;
     // Pi^neq = sum f c c - rho (u u + I/3) = sum g c c - rho u u - (rho-1)/3 I
     const T dr3 = dr / (T)3;
     m.pxx = sxx - rho * ux * ux - dr3;
@@ -423,10 +441,10 @@ __device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<
     T om = p.omega;
     if (RHEO != 0) {
         // |Pi^neq + sym(u F)| (src/CUDA/lbm.cu:192-208)
-        const T axx = m.pxx + ux * p.fx, ayy = m.pyy + uy * p.fy, azz = m.pzz + uz * p.fz;
-        const T axy = m.pxy + (ux * p.fy + uy * p.fx) * (T)0.5;
-        const T axz = m.pxz + (ux * p.fz + uz * p.fx) * (T)0.5;
-        const T ayz = m.pyz + (uy * p.fz + uz * p.fy) * (T)0.5;
+        const T axx = m.pxx + ux * F.x, ayy = m.pyy + uy * F.y, azz = m.pzz + uz * F.z;
+        const T axy = m.pxy + (ux * F.y + uy * F.x) * (T)0.5;
+        const T axz = m.pxz + (ux * F.z + uz * F.x) * (T)0.5;
+        const T ayz = m.pyz + (uy * F.z + uz * F.y) * (T)0.5;
         const T smag = sqrt((T)0.5 * (axx * axx + ayy * ayy + azz * azz + 2 * (axy * axy + axz * axz + ayz * ayz)));
         if (RHEO == 1) {
             // Bingham: omega = omega_p max(0, 1 - tau_y/|S|) (src/CUDA/NNF/nnf.h:83);
@@ -460,11 +478,11 @@ __device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<
 //        + (1-omega/2) w_i (3 c.F + 9 (c.u)(c.F) - 3 u.F)
 // with Q_i = c_i c_i - I/3 (src/CUDA/lbm.cu:263-461).
 template <int Q, int RHEO, typename T>
-__device__ __forceinline__ void collide(T* g, const Moments<Q, RHEO, T>& m, const Params<T>& p) {
+__device__ __forceinline__ void collide(T* g, const Moments<Q, RHEO, T>& m, const Force<T>& F) {
     const T om = m.omega;
     const T a = (T)4.5 * ((T)1 - om);
     const T b = (T)1 - (T)0.5 * om;
-    const T uf = m.ux * p.fx + m.uy * p.fy + m.uz * p.fz;
+    const T uf = m.ux * F.x + m.uy * F.y + m.uz * F.z;
     const T usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
     const T tr = m.pxx + m.pyy + m.pzz;
     const T k0 = m.dr - (T)1.5 * m.rho * usq - a * tr / (T)3 - (T)3 * b * uf;
@@ -475,7 +493,7 @@ __device__ __forceinline__ void collide(T* g, const Moments<Q, RHEO, T>& m, cons
     for (int i = 0; i < Q; i++) {
         const int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
         const T cu = (T)cx * m.ux + (T)cy * m.uy + (T)cz * m.uz;
-        const T cf = (T)cx * p.fx + (T)cy * p.fy + (T)cz * p.fz;
+        const T cf = (T)cx * F.x + (T)cy * F.y + (T)cz * F.z;
         T cpc = 0;
         if (cx) cpc += m.pxx;
         if (cy) cpc += m.pyy;
@@ -494,8 +512,10 @@ __device__ __forceinline__ void collide(T* g, const Moments<Q, RHEO, T>& m, cons
 constexpr int kBlock = 256;
 
 // One time step. DIRECT: the source holds post-stream populations (first step
-// after init / load), read in place without streaming or BC.
-template <int Q, typename S, int RHEO, bool MASK, bool DIRECT>
+// after init / load), read in place without streaming or BC. IBM: the force of
+// nodes in flagged tiles comes from the IBM force field (src/CUDA/lbm.cu:45-58
+// reads macr.f per node in IBM builds).
+template <int Q, typename S, int RHEO, bool MASK, bool DIRECT, bool IBM = false>
 __global__ __launch_bounds__(kBlock) void k_step(const Params<typename ComputeOf<S>::type> p) {
     using T = typename ComputeOf<S>::type;
     const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
@@ -510,14 +530,16 @@ __global__ __launch_bounds__(kBlock) void k_step(const Params<typename ComputeOf
         gather<Q, S, T>(g, p, nd);
         if (w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
     }
+    Force<T> F = const_force(p);
+    if (IBM && p.tile_flag[tile_of(p, nd.x, nd.y, nd.z)]) F = Force<T>{p.ibm_fx[idx], p.ibm_fy[idx], p.ibm_fz[idx]};
     const T om_old = (RHEO == 2) ? p.omega_state[idx] : (T)0;
-    const Moments<Q, RHEO, T> m = moments<Q, RHEO, T>(g, p, om_old);
+    const Moments<Q, RHEO, T> m = moments<Q, RHEO, T>(g, p, om_old, F);
     if (RHEO == 2) p.omega_state[idx] = m.omega;
     if (p.save) {
         p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
         if (RHEO != 0) p.m_omega[idx] = m.omega;
     }
-    collide<Q, RHEO, T>(g, m, p);
+    collide<Q, RHEO, T>(g, m, F);
     S* dst = (S*)p.dst;
 #pragma unroll
     for (int i = 0; i < Q; i++) st_nt<S, T>(dst + (int64_t)i * p.stride + idx, g[i]);
@@ -532,6 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<
     const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
     if (idx >= p.n) return;
     const Node<T> nd = make_node(p, idx);
+    if (p.aux_mode == kAuxMomentsOutsideTiles && p.tile_flag[tile_of(p, nd.x, nd.y, nd.z)]) return;
     const uint32_t word = node_word<MASK>(p, nd);
     T g[Q];
     if (p.direct || !w_used(word)) {
@@ -548,9 +571,48 @@ __global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<
     }
     if (!w_used(word)) return;
     const T om_old = (RHEO == 2) ? p.omega_state[idx] : (T)0;
-    const Moments<Q, RHEO, T> m = moments<Q, RHEO, T>(g, p, om_old);
+    const Moments<Q, RHEO, T> m = moments<Q, RHEO, T>(g, p, om_old, const_force(p));
     p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
     if (RHEO != 0) p.m_omega[idx] = m.omega;
+}
+
+// IBM macroscopics on the flagged tiles (gpuUpdateMacrIBM, src/CUDA/IBM/ibm.cu:544-648):
+// rho and u of the post-step populations with the constant force; the force
+// field of those nodes restarts from the constant force. Grid-stride over the
+// tile list (its length is read on the device).
+template <int Q, typename S, bool MASK>
+__global__ __launch_bounds__(kBlock) void k_ibm_macr(const Params<typename ComputeOf<S>::type> p,
+                                                     const uint32_t* tile_list, const uint32_t* tile_count) {
+    using T = typename ComputeOf<S>::type;
+    const uint32_t count = *tile_count;
+    T* ffx = (T*)p.ibm_fx;
+    T* ffy = (T*)p.ibm_fy;
+    T* ffz = (T*)p.ibm_fz;
+    for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
+        const uint32_t tile = tile_list[t];
+        const int tx = (int)(tile % (uint32_t)p.tiles_x);
+        const int ty = (int)((tile / (uint32_t)p.tiles_x) % (uint32_t)p.tiles_y);
+        const int tz = (int)(tile / ((uint32_t)p.tiles_x * (uint32_t)p.tiles_y));
+        for (int l = threadIdx.x; l < kTileEdge * kTileEdge * kTileEdge; l += kBlock) {
+            const int x = (tx << kTileShift) + (l & (kTileEdge - 1));
+            const int y = (ty << kTileShift) + ((l >> kTileShift) & (kTileEdge - 1));
+            const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
+            if (x >= p.nx || y >= p.ny || z >= p.nz) continue;
+            const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+            const Node<T> nd = make_node(p, idx);
+            const uint32_t word = node_word<MASK>(p, nd);
+            T g[Q];
+            if (p.direct || !w_used(word)) {
+                load_direct<Q, S, T>(g, p, nd);
+            } else {
+                gather<Q, S, T>(g, p, nd);
+                if (w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
+            }
+            const Moments<Q, 0, T> m = moments<Q, 0, T>(g, p, (T)0, const_force(p));
+            p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
+            ffx[idx] = p.fx; ffy[idx] = p.fy; ffz[idx] = p.fz;
+        }
+    }
 }
 
 // Equilibrium deviations from (rho, u) (gpu_f_eq, src/CUDA/globalFunctions.h:50-56).

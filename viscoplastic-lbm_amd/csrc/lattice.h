@@ -124,12 +124,24 @@ struct Params {
     int aux_mode;               // aux kernel: kAuxMoments / kAuxMaterialize
     int direct;                 // aux kernel: state is post-stream (no gather, no BC)
     T duct_R, duct_r, w_out, u_out, w_in, u_in;
+    // immersed boundary: nodes in 8^3 tiles flagged by the last IBM stage take
+    // their force from ibm_f*, every other node the constant (fx, fy, fz)
+    const uint8_t* tile_flag;
+    const T* ibm_fx; const T* ibm_fy; const T* ibm_fz;
+    int tiles_x, tiles_y;
 };
 
-enum AuxMode : int { kAuxMoments = 0, kAuxMaterialize = 1 };
+// Tiles of the IBM stage: 8 x 8 x 8 nodes.
+constexpr int kTileShift = 3;
+constexpr int kTileEdge = 1 << kTileShift;
+
+// kAuxMomentsOutsideTiles: moments with the constant force for nodes outside
+// the flagged tiles; flagged tiles keep the macroscopics of the IBM stage.
+enum AuxMode : int { kAuxMoments = 0, kAuxMaterialize = 1, kAuxMomentsOutsideTiles = 2 };
 
 // Launchers implemented in kernels.hip. prec: 8, 4 or 2 (storage bytes).
-int launch_step(int q, int prec, int rheo, bool mask_geom, bool direct, const void* params, void* stream);
+int launch_step(int q, int prec, int rheo, bool mask_geom, bool direct, const void* params, void* stream,
+                bool ibm = false);
 int launch_aux(int q, int prec, int rheo, bool mask_geom, const void* params, void* stream);
 int launch_reduce(int prec, const void* rho, const void* ux, const void* uy, const void* uz, void* ux_old,
                   void* uy_old, void* uz_old, uint32_t n, double* partials, double* out3, void* stream);

@@ -40,6 +40,29 @@ class VlbmDesc(C.Structure):
     ]
 
 
+class VlbmIbmDesc(C.Structure):
+    """Mirror of vlbm_ibm_desc (include/vlbm.h)."""
+
+    _fields_ = [
+        ("max_iterations", C.c_int),
+        ("periodic_x", C.c_int), ("periodic_y", C.c_int), ("periodic_z", C.c_int),
+        ("thickness", C.c_double), ("alpha", C.c_double), ("fluid_density", C.c_double),
+        ("gx", C.c_double), ("gy", C.c_double), ("gz", C.c_double),
+        ("pp_friction", C.c_double), ("pw_friction", C.c_double),
+        ("pp_rest", C.c_double), ("pw_rest", C.c_double),
+        ("young", C.c_double), ("poisson", C.c_double),
+        ("wall_young", C.c_double), ("wall_poisson", C.c_double),
+        ("ref_float_helpers", C.c_int),
+        ("mesh_scale", C.c_double), ("breugem", C.c_double), ("coulomb", C.c_int),
+        ("max_spheres", C.c_int), ("max_nodes", C.c_int),
+    ]
+
+
+MESH_POLAR, MESH_FIBONACCI = 0, 1
+IBM_RECORD = 48
+IBM_HISTORY = 13
+
+
 class VlbmError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"vlbm error {code} ({ERRORS.get(code, '?')}): {msg}")
@@ -75,6 +98,22 @@ def _load() -> C.CDLL:
         "vlbm_time_steps": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "vlbm_sync": (C.c_int, [C.c_void_p]),
         "vlbm_device_bytes": (C.c_int, [C.c_void_p, P(C.c_size_t)]),
+        "vlbm_ibm_desc_defaults": (None, [P(VlbmIbmDesc)]),
+        "vlbm_ibm_enable": (C.c_int, [C.c_void_p, P(VlbmIbmDesc)]),
+        "vlbm_ibm_add_sphere": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                          P(C.c_int)]),
+        "vlbm_ibm_add_particle": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp,
+                                            P(C.c_int)]),
+        "vlbm_ibm_counts": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
+        "vlbm_ibm_get_particles": (C.c_int, [C.c_void_p, dp]),
+        "vlbm_ibm_set_particles": (C.c_int, [C.c_void_p, dp]),
+        "vlbm_ibm_get_collisions": (C.c_int, [C.c_void_p, P(C.c_int), dp, P(C.c_int)]),
+        "vlbm_ibm_get_nodes": (C.c_int, [C.c_void_p, dp, dp, dp]),
+        "vlbm_ibm_get_force": (C.c_int, [C.c_void_p, dp]),
+        "vlbm_ibm_cells": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
+        "vlbm_ibm_stage": (C.c_int, [C.c_void_p]),
+        "vlbm_ibm_status": (C.c_int, [C.c_void_p]),
+        "vlbm_ibm_time_steps": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -94,4 +133,14 @@ def check(code: int) -> None:
 def default_desc() -> VlbmDesc:
     d = VlbmDesc()
     lib.vlbm_desc_defaults(C.byref(d))
+    return d
+
+
+def default_ibm_desc(**kw) -> VlbmIbmDesc:
+    d = VlbmIbmDesc()
+    lib.vlbm_ibm_desc_defaults(C.byref(d))
+    for k, v in kw.items():
+        if not hasattr(d, k):
+            raise KeyError(f"unknown IBM descriptor field {k!r}")
+        setattr(d, k, v)
     return d

@@ -148,3 +148,78 @@ class Lattice:
     def save_info(self, path: str, sim_id: str, total_steps: int, mlups: float, seconds: float) -> None:
         check(lib.vlbm_save_info(self._h, path.encode(), sim_id.encode(), int(total_steps),
                                  float(mlups), float(seconds)))
+
+    # -- immersed boundary stage (src/CUDA/IBM/) --------------------------------
+    def ibm_enable(self, idesc: "_capi.VlbmIbmDesc") -> None:
+        check(lib.vlbm_ibm_enable(self._h, C.byref(idesc)))
+
+    def ibm_add_sphere(self, pos, vel=(0, 0, 0), w=(0, 0, 0), radius=1.0, density=2.0, movable=True,
+                       mesh=_capi.MESH_POLAR, n_fibonacci=0) -> int:
+        state = _f64(np.concatenate([pos, vel, w]))
+        sid = C.c_int(-1)
+        check(lib.vlbm_ibm_add_sphere(self._h, _ptr(state), float(radius), float(density), int(bool(movable)),
+                                      int(mesh), int(n_fibonacci), C.byref(sid)))
+        return sid.value
+
+    def ibm_add_particle(self, pos, vel, w, radius, density, movable, xyz, area) -> int:
+        state = _f64(np.concatenate([pos, vel, w]))
+        xyz = _f64(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+        area = _f64(area)
+        sid = C.c_int(-1)
+        check(lib.vlbm_ibm_add_particle(self._h, _ptr(state), float(radius), float(density), int(bool(movable)),
+                                        len(area), _ptr(xyz), _ptr(area), C.byref(sid)))
+        return sid.value
+
+    def ibm_counts(self) -> Tuple[int, int]:
+        ns, nn = C.c_int(0), C.c_int(0)
+        check(lib.vlbm_ibm_counts(self._h, C.byref(ns), C.byref(nn)))
+        return ns.value, nn.value
+
+    def ibm_particles(self) -> np.ndarray:
+        ns, _ = self.ibm_counts()
+        out = np.zeros((ns, _capi.IBM_RECORD))
+        if ns:
+            check(lib.vlbm_ibm_get_particles(self._h, _ptr(out)))
+        return out
+
+    def ibm_set_particles(self, rec: np.ndarray) -> None:
+        check(lib.vlbm_ibm_set_particles(self._h, _ptr(_f64(rec))))
+
+    def ibm_collision_state(self):
+        ns, _ = self.ibm_counts()
+        partner = np.zeros((ns, _capi.IBM_HISTORY), dtype=np.int32)
+        last = np.zeros((ns, _capi.IBM_HISTORY), dtype=np.int32)
+        tang = np.zeros((ns, _capi.IBM_HISTORY, 3))
+        ip = C.POINTER(C.c_int)
+        check(lib.vlbm_ibm_get_collisions(self._h, partner.ctypes.data_as(ip), _ptr(tang), last.ctypes.data_as(ip)))
+        return partner, tang, last
+
+    def ibm_nodes(self):
+        _, nn = self.ibm_counts()
+        pos, f, df = np.zeros((3, nn)), np.zeros((3, nn)), np.zeros((3, nn))
+        check(lib.vlbm_ibm_get_nodes(self._h, _ptr(pos), _ptr(f), _ptr(df)))
+        return pos, f, df
+
+    def ibm_force(self) -> np.ndarray:
+        out = np.zeros((3,) + self.shape)
+        check(lib.vlbm_ibm_get_force(self._h, _ptr(out)))
+        return out
+
+    def ibm_cells(self):
+        ns, _ = self.ibm_counts()
+        dims = np.zeros(3, dtype=np.int32)
+        cells = np.zeros(max(ns, 1), dtype=np.int32)
+        ip = C.POINTER(C.c_int)
+        check(lib.vlbm_ibm_cells(self._h, dims.ctypes.data_as(ip), cells.ctypes.data_as(ip)))
+        return dims, cells[:ns]
+
+    def ibm_stage(self) -> None:
+        check(lib.vlbm_ibm_stage(self._h))
+
+    def ibm_status(self) -> None:
+        check(lib.vlbm_ibm_status(self._h))
+
+    def ibm_time_steps(self, n: int) -> Tuple[float, float]:
+        a, b = C.c_double(0), C.c_double(0)
+        check(lib.vlbm_ibm_time_steps(self._h, int(n), C.byref(a), C.byref(b)))
+        return a.value, b.value
