@@ -258,6 +258,63 @@ int vlbm_ibm_status(vlbm_h h);
  * IBM stages separately (HIP events around each part). */
 int vlbm_ibm_time_steps(vlbm_h h, int n_steps, double* ms_lbm, double* ms_ibm);
 
+/* ------------------------------------------------------------------------
+ * Serving: many independent lattices resident on one GPU, admitted by an
+ * HBM budget and stepped together by one persistent-grid kernel over all of
+ * them, with captured step sequences (hipGraph) and an autotuned launch
+ * shape. No reference counterpart: the reference runs one compile-time-sized
+ * simulation per process (src/CUDA/main.cu). Jobs use the box builders and
+ * start from rest (rho0, u = 0); the stop test is the reference's
+ * (treatData / stopSim, src/CUDA/treatData.cpp:25-78,451-459).
+ * ------------------------------------------------------------------------ */
+typedef struct vlbm_batch_desc {
+    size_t hbm_budget;       /* bytes of the arena that holds the jobs' lattices */
+    int max_jobs;            /* resident jobs at most */
+    int q, precision;        /* shared by every job of the batch */
+    int check_every;         /* steps between residual checks of a job (even) */
+    int steps_per_launch;    /* steps per captured launch sequence; 0 = autotune (even, divides check_every) */
+    int block;               /* threads per block, 128 or 256; 0 = autotune */
+    int grid;                /* blocks of the persistent grid; 0 = autotune */
+    int use_graph;           /* 1: replay captured sequences (hipGraph); 0: plain launches */
+    const char* tune_cache;  /* file the autotuner reads and appends (NULL: none) */
+} vlbm_batch_desc;
+
+#define VLBM_JOB_PENDING 0
+#define VLBM_JOB_RUNNING 1
+#define VLBM_JOB_DONE 2          /* residual < resid_max */
+#define VLBM_JOB_MAX_STEPS 3     /* reached max_steps (checked at its check points) */
+#define VLBM_JOB_FAILED 4        /* non-finite or negative mean density */
+
+typedef struct vlbm_job_info {
+    int state;               /* VLBM_JOB_* */
+    long long steps;
+    double residual, mean_rho;
+    double t_submit, t_start, t_end;   /* seconds on the batch clock; -1 = not yet */
+    size_t bytes;            /* arena bytes while resident */
+} vlbm_job_info;
+
+typedef struct vlbm_batch* vlbm_batch_h;
+
+void vlbm_batch_desc_defaults(vlbm_batch_desc* d);
+int vlbm_batch_create(const vlbm_batch_desc* d, vlbm_batch_h* out);
+void vlbm_batch_destroy(vlbm_batch_h b);
+/* Arena bytes a job would hold (two lattices, previous velocity, power-law omega). */
+int vlbm_batch_job_bytes(vlbm_batch_h b, const vlbm_desc* d, size_t* bytes);
+/* Queues a job; it is admitted by vlbm_batch_advance when a slot and memory are free. */
+int vlbm_batch_submit(vlbm_batch_h b, const vlbm_desc* d, double resid_max, long long max_steps, int* job);
+/* Admits what fits, runs one chunk of steps of every resident job, checks the
+ * jobs at their check points and retires the finished ones. */
+int vlbm_batch_advance(vlbm_batch_h b, int* n_running, int* n_pending, int* n_finished);
+int vlbm_batch_job_info(vlbm_batch_h b, int job, vlbm_job_info* info);
+/* Final macroscopics of a finished job, [NZ][NY][NX] doubles each (any may be NULL). */
+int vlbm_batch_job_result(vlbm_batch_h b, int job, double* rho, double* ux, double* uy, double* uz);
+/* Drops the host copy of a finished job's result. */
+int vlbm_batch_release(vlbm_batch_h b, int job);
+/* Launch shape in use (after autotuning). */
+int vlbm_batch_config(vlbm_batch_h b, int* steps_per_launch, int* block, int* grid);
+/* Seconds since vlbm_batch_create (the clock of vlbm_job_info). */
+double vlbm_batch_clock(vlbm_batch_h b);
+
 #ifdef __cplusplus
 }
 #endif

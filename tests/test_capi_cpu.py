@@ -111,3 +111,73 @@ def test_yaml_surface(tmp_path):
     bad.write_text(text.replace("precision: single", "precision: quad"))
     with pytest.raises(ValueError, match="Unable to load YAML"):
         vlbm.read_configs(str(bad))
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """ctypes mirrors have the sizes and field offsets of include/vlbm.h."""
+    import subprocess
+    from vlbm import serve
+    structs = {"vlbm_desc": _capi.VlbmDesc, "vlbm_ibm_desc": _capi.VlbmIbmDesc,
+               "vlbm_batch_desc": serve.VlbmBatchDesc, "vlbm_job_info": serve.VlbmJobInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vlbm.h"', 'int main(void) {']
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                         check=True).stdout.split("\n") if l)
+    for cname, cls in structs.items():
+        assert int(got[cname]) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_ibm_and_batch_descriptor_defaults():
+    d = _capi.default_ibm_desc()
+    # ibmVar.h / ibmBoundaryCondition.h constants
+    assert (d.max_iterations, d.thickness, d.alpha, d.fluid_density) == (1, 1.0, 0.5, 1.0)
+    assert (d.periodic_x, d.periodic_y, d.periodic_z) == (1, 0, 1)
+    assert (d.pp_friction, d.pw_friction, d.pp_rest, d.pw_rest) == (0.1, 0.1, 0.9, 0.9)
+    assert (d.young, d.poisson, d.wall_young, d.wall_poisson) == (0.5, 0.3, 100.0, 0.3)
+    from vlbm import serve
+    b = serve.batch_desc()
+    assert b.q == 19 and b.precision == _capi.FP32 and b.use_graph == 1 and b.check_every % 2 == 0
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(max_jobs=0), "max_jobs"),
+    (dict(check_every=7), "check_every"),
+    (dict(check_every=100, steps_per_launch=30), "steps_per_launch"),
+    (dict(block=64), "block"),
+    (dict(q=15), "velocity set"),
+])
+def test_batch_rejects_bad_descriptors(kw, msg):
+    """Checked before any device call, so this runs without a GPU."""
+    from vlbm import serve
+    with pytest.raises(vlbm.VlbmError, match=msg):
+        serve.Batch(**kw)
+
+
+def test_sweep_job_generator_and_bingham_number():
+    from vlbm import serve
+    jobs = serve.make_jobs(50, rate=20.0, seed=3)
+    arr = np.array([j.arrival for j in jobs])
+    assert np.all(np.diff(arr) > 0)
+    assert 1.0 < arr[-1] < 5.0                       # ~50 / 20 s
+    for j in jobs:
+        assert 32 <= j.nx <= 256 and 32 <= j.ny <= 128 and 16 <= j.nz <= 128
+        assert 0.55 <= j.tau <= 1.5 and 1e-6 <= j.fx <= 1e-5 and 0 <= j.bn <= 5
+        h = j.ny / 2.0
+        nu = (j.tau - 0.5) / 3.0
+        assert j.tau_y < j.fx * h                    # the plug stays inside the channel
+        if j.bn > 0:
+            U = serve.mean_velocity(h, j.fx, nu, j.tau_y)
+            assert abs(j.tau_y * 2 * h / (nu * U) - j.bn) < 1e-6 * max(1.0, j.bn)
+    # Newtonian limit: the profile is the Poiseuille parabola
+    s = np.linspace(-4, 4, 9)
+    assert np.allclose(serve.bingham_profile(s, 4.0, 1e-5, 0.1, 0.0), 1e-5 / 0.2 * (16 - s * s))

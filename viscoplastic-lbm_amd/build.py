@@ -36,6 +36,8 @@ SOURCES = [
     "kernels_common.hip",
     "kernels_ibm.hip",
     "ibm_capi.cpp",
+    "kernels_batch.hip",
+    "batch_capi.cpp",
     "capi.cpp",
 ]
 

@@ -234,6 +234,18 @@ int build_words(const vlbm_desc& d, std::vector<uint32_t>& out) {
 
 }  // namespace
 
+namespace vlbm {
+namespace detail {
+
+int validate_desc(const vlbm_desc* d) { return validate(d); }
+
+bool box_builder_words(int builder, uint32_t out[27]) {
+    return is_box_builder(builder) && box_region_words(builder, out);
+}
+
+}  // namespace detail
+}  // namespace vlbm
+
 extern "C" {
 
 const char* vlbm_last_error(void) { return g_err.c_str(); }

@@ -58,7 +58,7 @@ struct vlbm_ctx {
     size_t device_bytes = 0;
     vlbm::IbmState* ibm = nullptr;   // immersed boundary stage (vlbm_ibm_enable)
     // what the step kernel needs of the IBM stage (null without IBM)
-    const uint8_t* tile_flag = nullptr;
+    const uint32_t* tile_flag = nullptr;
     void* ibm_force[3] = {nullptr, nullptr, nullptr};
     int tiles_x = 0, tiles_y = 0;
 };
@@ -66,17 +66,19 @@ struct vlbm_ctx {
 namespace vlbm {
 namespace detail {
 
+int validate_desc(const vlbm_desc* d);                 // capi.cpp
+bool box_builder_words(int builder, uint32_t out[27]);  // capi.cpp: node word per box region
+
+// Physics and geometry part of the kernel parameters (no buffers).
 template <typename T>
-inline void fill_params(const vlbm_ctx* h, Params<T>& p) {
+inline void fill_params_desc(const vlbm_desc& d, const uint32_t region_word[27], Params<T>& p) {
     std::memset(&p, 0, sizeof(p));
-    const vlbm_desc& d = h->d;
     p.nx = d.nx; p.ny = d.ny; p.nz = d.nz;
-    p.n = (uint32_t)h->n;
+    p.n = (uint32_t)((size_t)d.nx * d.ny * d.nz);
     p.div_nx = make_fastdiv((uint32_t)d.nx);
     p.div_nxy = make_fastdiv((uint32_t)(d.nx * d.ny));
-    p.stride = h->stride;
-    p.mask = h->d_mask;
-    for (int r = 0; r < 27; r++) p.region_word[r] = h->region_word[r];
+    p.stride = (int64_t)p.n;
+    for (int r = 0; r < 27; r++) p.region_word[r] = region_word[r];
     for (int k = 0; k < 8; k++) {
         p.ux_bc[k] = (T)d.ux_bc[k]; p.uy_bc[k] = (T)d.uy_bc[k];
         p.uz_bc[k] = (T)d.uz_bc[k]; p.rho_bc[k] = (T)d.rho_bc[k];
@@ -90,13 +92,20 @@ inline void fill_params(const vlbm_ctx* h, Params<T>& p) {
     p.gamma_0 = (T)d.gamma_0;
     p.clamp = d.omega_clamp;
     if (d.omega_clamp) { p.omega_lo = (T)(1.0 / d.tau_max); p.omega_hi = (T)(1.0 / d.tau_min); }
-    p.omega_state = (T*)h->omega_state;
-    p.m_rho = (T*)h->macr[0]; p.m_ux = (T*)h->macr[1]; p.m_uy = (T*)h->macr[2];
-    p.m_uz = (T*)h->macr[3]; p.m_omega = (T*)h->macr[4];
     p.zouhe_textbook = d.zouhe_textbook;
     p.duct_R = (T)d.duct_outer_radius; p.duct_r = (T)d.duct_inner_radius;
     p.w_out = (T)d.outer_rotation; p.u_out = (T)d.outer_velocity;
     p.w_in = (T)d.inner_rotation; p.u_in = (T)d.inner_velocity;
+}
+
+template <typename T>
+inline void fill_params(const vlbm_ctx* h, Params<T>& p) {
+    fill_params_desc(h->d, h->region_word, p);
+    p.stride = h->stride;
+    p.mask = h->d_mask;
+    p.omega_state = (T*)h->omega_state;
+    p.m_rho = (T*)h->macr[0]; p.m_ux = (T*)h->macr[1]; p.m_uy = (T*)h->macr[2];
+    p.m_uz = (T*)h->macr[3]; p.m_omega = (T*)h->macr[4];
     p.tile_flag = h->tile_flag;
     p.ibm_fx = (const T*)h->ibm_force[0]; p.ibm_fy = (const T*)h->ibm_force[1]; p.ibm_fz = (const T*)h->ibm_force[2];
     p.tiles_x = h->tiles_x; p.tiles_y = h->tiles_y;

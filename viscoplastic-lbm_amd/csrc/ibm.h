@@ -6,6 +6,7 @@
 // is FP64; the fluid-side fields (macroscopics, force field, spread force) are
 // in the lattice's compute type.
 #pragma once
+#include <cmath>
 #include <cstdint>
 
 namespace vlbm {
@@ -60,15 +61,15 @@ struct IbmParams {
     double *pdx, *pdy, *pdz;               // force of the last iteration
     double *pmx, *pmy, *pmz;               // its moment about the sphere centre
     int* pid;                              // sphere of each node
-    uint8_t* tile_flag;
-    uint32_t* tile_list;
+    uint32_t* tile_flag;                   // 1: the tile is in tile_list
+    uint32_t* tile_list;                   // flagged tiles, in no particular order
     uint32_t* tile_count;
     int cells[3];
     double cell_w[3];
-    uint64_t* cell_keys;                   // (cell << 32) | sphere, sorted ascending
     IbmContact* contacts;                  // [n_spheres][kMaxContacts]
     int* n_contacts;
     int* error;
+    int box_edge;                          // LDS spreading box edge (0: global atomics)
 };
 
 template <typename T>
@@ -86,11 +87,15 @@ int launch_ibm_stage(const IbmParams& P, int tbytes, const void* fluid, int stag
 
 // Parts of one stage, enqueued in this order (immersedBoundaryMethod, ibm.cu:6-192).
 enum IbmStagePart : int {
-    kIbmBegin = 0,        // old values, cell index, node force reset, tile flags
-    kIbmTiles = 1,        // compaction of the flagged tiles
-    kIbmContacts = 2,     // cell sort, contacts, force and torque sums, velocity update
-    kIbmIteration = 3,    // interpolation / spreading, sphere sums, velocity update, Euler sum
-    kIbmMove = 4,         // sphere and node movement
+    kIbmBegin = 0,        // old values, cell index, node force reset, tile list
+    kIbmContacts = 1,     // cell sort, contacts, force and torque sums, velocity update
+    kIbmIteration = 2,    // interpolation / spreading, sphere sums, velocity update, Euler sum
+    kIbmMove = 3,         // sphere and node movement
 };
+
+// Edge of the LDS box that collects one sphere's spread force (FP32 fluids):
+// the 4^3 stencils of nodes within the radius of the centre fit in
+// 2 ceil(r) + 6 nodes per axis.
+inline int ibm_box_edge(double r_max) { return 2 * (int)std::ceil(r_max) + 6; }
 
 }  // namespace vlbm

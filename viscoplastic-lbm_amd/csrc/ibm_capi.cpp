@@ -42,6 +42,17 @@ int ibm_alloc(vlbm_ctx* h, void** p, size_t bytes) {
 // Uniform cell list: cells at least one contact distance (2 r_max, with a
 // margin for the float gap test) wide.
 void update_cells(const vlbm_ctx* h, IbmState& s) {
+    // LDS spreading box for FP32 fluids when it fits the workgroup's LDS
+    s.P.box_edge = 0;
+    if (h->tbytes == 4 && s.r_max > 0) {
+        const int edge = ibm_box_edge(s.r_max);
+        const size_t bytes = 3 * sizeof(float) * (size_t)edge * edge * edge;
+        hipDeviceProp_t prop;
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+            bytes + 4096 <= prop.sharedMemPerBlock)
+            s.P.box_edge = edge;
+    }
     const int n[3] = {h->d.nx, h->d.ny, h->d.nz};
     const double cs = 2.0 * s.r_max * 1.0001;
     for (int a = 0; a < 3; a++) {
@@ -252,10 +263,9 @@ int ibm_stage(vlbm_ctx* h) {
     IbmState& s = *h->ibm;
     s.P.step = (int)h->steps;
     int rc = stage_part(h, kIbmBegin);
-    if (!rc) rc = stage_part(h, kIbmTiles);
     if (rc) return rc;
     const vlbm_desc& d = h->d;
-    const uint32_t max_tiles = s.P.n_tiles < 2048u ? s.P.n_tiles : 2048u;
+    const uint32_t max_tiles = s.P.n_tiles;
     rc = with_params(h, [&](void* pv, auto& p) {
         p.src = h->lat[h->cur];
         p.dst = h->lat[1 - h->cur];
@@ -286,7 +296,8 @@ int ibm_fill_macr(vlbm_ctx* h) {
 
 int ibm_reset_tiles(vlbm_ctx* h) {
     if (!h->ibm) return VLBM_OK;
-    HIP_TRY(hipMemsetAsync(h->ibm->P.tile_flag, 0, h->ibm->P.n_tiles, h->stream));
+    HIP_TRY(hipMemsetAsync(h->ibm->P.tile_flag, 0, (size_t)h->ibm->P.n_tiles * 4, h->stream));
+    HIP_TRY(hipMemsetAsync(h->ibm->P.tile_count, 0, 4, h->stream));
     return sync(h);
 }
 
@@ -356,10 +367,9 @@ int vlbm_ibm_enable(vlbm_h h, const vlbm_ibm_desc* d) {
     double** dn[13] = {&P.px, &P.py, &P.pz, &P.pS, &P.pfx, &P.pfy, &P.pfz, &P.pdx, &P.pdy, &P.pdz, &P.pmx, &P.pmy, &P.pmz};
     for (double** q : dn) A((void**)q, mn * 8);
     A((void**)&P.pid, mn * 4);
-    A((void**)&P.tile_flag, P.n_tiles);
+    A((void**)&P.tile_flag, (size_t)P.n_tiles * 4);
     A((void**)&P.tile_list, (size_t)P.n_tiles * 4);
     A((void**)&P.tile_count, 4);
-    A((void**)&P.cell_keys, ms * 8);
     A((void**)&P.contacts, ms * kMaxContacts * sizeof(IbmContact));
     A((void**)&P.n_contacts, ms * 4);
     A((void**)&P.error, 4);
@@ -373,7 +383,7 @@ int vlbm_ibm_enable(vlbm_h h, const vlbm_ibm_desc* d) {
         e = hipMemsetAsync(s.spread[a], 0, h->n * h->tbytes, h->stream);
         if (e == hipSuccess) e = hipMemsetAsync(s.force[a], 0, h->n * h->tbytes, h->stream);
     }
-    if (e == hipSuccess) e = hipMemsetAsync(P.tile_flag, 0, P.n_tiles, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(P.tile_flag, 0, (size_t)P.n_tiles * 4, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(P.tile_count, 0, 4, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(P.error, 0, 4, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(P.n_contacts, 0, ms * 4, h->stream);
@@ -478,8 +488,8 @@ int vlbm_ibm_get_force(vlbm_h h, double* f) {
     if (rc) return rc;
     if (!f) return fail(VLBM_ERR_INVALID, "output is NULL");
     const IbmState& s = *h->ibm;
-    std::vector<uint8_t> flags(s.P.n_tiles);
-    HIP_TRY(hipMemcpyAsync(flags.data(), s.P.tile_flag, flags.size(), hipMemcpyDeviceToHost, h->stream));
+    std::vector<uint32_t> flags(s.P.n_tiles);
+    HIP_TRY(hipMemcpyAsync(flags.data(), s.P.tile_flag, flags.size() * 4, hipMemcpyDeviceToHost, h->stream));
     if ((rc = sync(h))) return rc;
     const double f0[3] = {h->d.fx, h->d.fy, h->d.fz};
     for (int a = 0; a < 3; a++) {

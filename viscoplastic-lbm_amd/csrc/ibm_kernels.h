@@ -1,18 +1,20 @@
 // HIP kernels of the immersed boundary stage (rigid spheres), gfx950.
 //
 // One stage per time step, after the LBM step (src/CUDA/main.cu:369-389):
-//   begin     old values, cell index of each sphere, node forces reset, tiles
-//             touched by the 4^3 interpolation stencils flagged
-//   tiles     flagged 8^3 tiles compacted into a list (one block, fixed order)
+//   begin     old values, cell index of each sphere, node forces reset; the
+//             4^3 tiles touched by the interpolation stencils are appended to
+//             a tile list (first touch wins an atomic flag)
 //   macr      rho, u of the post-step populations with the constant force on
-//             the flagged tiles only; force field reset to the constant there
-//   contacts  uniform cell list (one-block bitonic sort of (cell, sphere)),
-//             Hertz-Mindlin sphere-sphere contacts by the lower-index sphere,
-//             force / torque sums per sphere in a fixed order, walls, velocity
-//   iteration (x max_iterations) interpolation + force of each node, spread
-//             with atomics into the tile force buffer, per-sphere FP64 sums of
-//             the node forces / moments (fixed-order tree), velocity update,
-//             Euler sum u += F/(2 rho), force field += F on the flagged tiles
+//             the listed tiles only; force field reset to the constant there
+//   contacts  one workgroup: uniform cell list (bitonic sort of (cell, sphere)
+//             in LDS), Hertz-Mindlin sphere-sphere contacts by the lower-index
+//             sphere, force / torque sums per sphere in a fixed order, walls,
+//             velocity update
+//   iteration (x max_iterations) one workgroup per sphere: interpolation and
+//             force of each node, spread into an LDS box around the sphere,
+//             the box added to the fluid's spread-force buffer, FP64 sums of
+//             the node forces / moments (fixed-order tree); velocity update;
+//             Euler sum u += F/(2 rho), force field += F on the listed tiles
 //   move      sphere and node movement
 // Everything outside the flagged tiles is left alone: the reference's full-
 // lattice passes (gpuUpdateMacrIBM, gpuEulerSumIBMAuxsReset over the bordered
@@ -101,6 +103,7 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_begin(IbmParams P) {
     const uint32_t gid = blockIdx.x * kIbmBlock + threadIdx.x;
     const uint32_t stride = gridDim.x * kIbmBlock;
     for (uint32_t t = gid; t < P.n_tiles; t += stride) P.tile_flag[t] = 0;
+    if (gid == 0) *P.tile_count = 0;
     if (gid >= (uint32_t)P.n_spheres) return;
     IbmSphere& c = P.sph[gid];
     int cc[3];
@@ -137,59 +140,15 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_mark(IbmParams P) {
     }
     for (int c = 0; c < nt[2]; c++)
         for (int b = 0; b < nt[1]; b++)
-            for (int a = 0; a < nt[0]; a++)
-                P.tile_flag[tl[0][a] + P.tiles[0] * (tl[1][b] + P.tiles[1] * tl[2][c])] = 1;
-}
-
-// flagged tiles -> list, ascending tile index (one block)
-__global__ __launch_bounds__(1024) void k_ibm_compact(IbmParams P) {
-    __shared__ uint32_t sums[1024];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t per = (P.n_tiles + 1023) / 1024;
-    const uint32_t beg = tid * per;
-    const uint32_t end = beg + per < P.n_tiles ? beg + per : P.n_tiles;
-    uint32_t cnt = 0;
-    for (uint32_t t = beg; t < end; t++) cnt += P.tile_flag[t] ? 1u : 0u;
-    sums[tid] = cnt;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = tid >= off ? sums[tid - off] : 0u;
-        __syncthreads();
-        sums[tid] += v;
-        __syncthreads();
-    }
-    uint32_t at = sums[tid] - cnt;
-    for (uint32_t t = beg; t < end; t++)
-        if (P.tile_flag[t]) P.tile_list[at++] = t;
-    if (tid == 1023) *P.tile_count = sums[1023];
+            for (int a = 0; a < nt[0]; a++) {
+                const uint32_t t = (uint32_t)(tl[0][a] + P.tiles[0] * (tl[1][b] + P.tiles[1] * tl[2][c]));
+                if (atomicExch(P.tile_flag + t, 1u) == 0u) P.tile_list[atomicAdd(P.tile_count, 1u)] = t;
+            }
 }
 
 // ---------------------------------------------------------------------------
 // cell list
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_ibm_sort_cells(IbmParams P) {
-    __shared__ uint64_t key[kMaxSortKeys];
-    const int n = P.n_spheres;
-    int m = 1;
-    while (m < n) m <<= 1;
-    for (int i = threadIdx.x; i < m; i += 1024)
-        key[i] = i < n ? (((uint64_t)(uint32_t)P.sph[i].cell << 32) | (uint32_t)i) : ~0ull;
-    __syncthreads();
-    for (int k = 2; k <= m; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < m; i += 1024) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const bool up = (i & k) == 0;
-                    const uint64_t a = key[i], b = key[ixj];
-                    if ((a > b) == up) { key[i] = b; key[ixj] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    for (int i = threadIdx.x; i < n; i += 1024) P.cell_keys[i] = key[i];
-}
-
 // cells whose spheres can touch a sphere of the given cell (distinct cells)
 __device__ __forceinline__ int neighbour_cells(const IbmParams& P, int cell, int out[27]) {
     int cc[3];
@@ -218,11 +177,11 @@ __device__ __forceinline__ int neighbour_cells(const IbmParams& P, int cell, int
     return n;
 }
 
-__device__ __forceinline__ int lower_bound_key(const IbmParams& P, uint64_t k) {
-    int lo = 0, hi = P.n_spheres;
+__device__ __forceinline__ int lower_bound_key(const uint64_t* keys, int n, uint64_t k) {
+    int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (P.cell_keys[mid] < k) lo = mid + 1;
+        if (keys[mid] < k) lo = mid + 1;
         else hi = mid;
     }
     return lo;
@@ -408,21 +367,19 @@ __device__ void wall_contact(const IbmParams& P, IbmSphere& c, const double wall
     M[2] += r_i * (n[0] * ft[1] - n[1] * ft[0]);
 }
 
-// sphere-sphere contacts of each sphere with higher-index spheres of the
+// sphere-sphere contacts of sphere i with higher-index spheres of the
 // neighbouring cells, in ascending partner order (the reference's pair order
 // for that sphere's history)
-__global__ __launch_bounds__(kIbmBlock) void k_ibm_contacts(IbmParams P) {
-    const int i = blockIdx.x * kIbmBlock + threadIdx.x;
-    if (i >= P.n_spheres) return;
+__device__ void sphere_contacts(const IbmParams& P, const uint64_t* keys, int i) {
     IbmSphere& ci = P.sph[i];
     int cand[kMaxContacts];
     int nc = 0;
     int cells[27];
     const int ncell = neighbour_cells(P, ci.cell, cells);
     for (int c = 0; c < ncell; c++) {
-        const uint64_t k0 = (uint64_t)(uint32_t)cells[c] << 32;
-        for (int k = lower_bound_key(P, k0); k < P.n_spheres && (P.cell_keys[k] >> 32) == (uint64_t)(uint32_t)cells[c]; k++) {
-            const int j = (int)(P.cell_keys[k] & 0xffffffffu);
+        const uint64_t cell = (uint64_t)(uint32_t)cells[c];
+        for (int k = lower_bound_key(keys, P.n_spheres, cell << 32); k < P.n_spheres && (keys[k] >> 32) == cell; k++) {
+            const int j = (int)(keys[k] & 0xffffffffu);
             if (j <= i) continue;
             const IbmSphere& cj = P.sph[j];
             if (!ci.movable && !cj.movable) continue;
@@ -456,11 +413,9 @@ __device__ void wall_contacts(const IbmParams& P, IbmSphere& c, double f[3], dou
 }
 
 // force / torque of the contacts in the reference's accumulation order for
-// each sphere: as the higher index (ascending partner), as the lower index
+// sphere p: as the higher index (ascending partner), as the lower index
 // (ascending partner), then the walls
-__global__ __launch_bounds__(kIbmBlock) void k_ibm_contact_sum(IbmParams P) {
-    const int p = blockIdx.x * kIbmBlock + threadIdx.x;
-    if (p >= P.n_spheres) return;
+__device__ void sphere_contact_sum(const IbmParams& P, const uint64_t* keys, int p) {
     IbmSphere& cp = P.sph[p];
     double f[3] = {cp.f[0], cp.f[1], cp.f[2]};
     double M[3] = {cp.M[0], cp.M[1], cp.M[2]};
@@ -469,9 +424,9 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_contact_sum(IbmParams P) {
     int cells[27];
     const int ncell = neighbour_cells(P, cp.cell, cells);
     for (int c = 0; c < ncell; c++) {
-        const uint64_t k0 = (uint64_t)(uint32_t)cells[c] << 32;
-        for (int k = lower_bound_key(P, k0); k < P.n_spheres && (P.cell_keys[k] >> 32) == (uint64_t)(uint32_t)cells[c]; k++) {
-            const int i = (int)(P.cell_keys[k] & 0xffffffffu);
+        const uint64_t cell = (uint64_t)(uint32_t)cells[c];
+        for (int k = lower_bound_key(keys, P.n_spheres, cell << 32); k < P.n_spheres && (keys[k] >> 32) == cell; k++) {
+            const int i = (int)(keys[k] & 0xffffffffu);
             if (i >= p) continue;
             const int nci = P.n_contacts[i];
             for (int r = 0; r < nci; r++) {
@@ -498,9 +453,7 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_contact_sum(IbmParams P) {
 }
 
 // gpuUpdateParticleCenterVelocityAndRotation (ibm.cu:755-897)
-__global__ __launch_bounds__(kIbmBlock) void k_ibm_velocity(IbmParams P) {
-    const int s = blockIdx.x * kIbmBlock + threadIdx.x;
-    if (s >= P.n_spheres) return;
+__device__ void sphere_velocity(const IbmParams& P, int s) {
     IbmSphere& c = P.sph[s];
     if (!c.movable) return;
     const double al = P.alpha;
@@ -552,6 +505,42 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_velocity(IbmParams P) {
     for (int a = 0; a < 3; a++) c.w[a] = wN[a];
     c.I[0] = N3[0][0]; c.I[1] = N3[1][1]; c.I[2] = N3[2][2];
     c.I[3] = N3[0][1]; c.I[4] = N3[0][2]; c.I[5] = N3[1][2];
+}
+
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_velocity(IbmParams P) {
+    const int s = blockIdx.x * kIbmBlock + threadIdx.x;
+    if (s < P.n_spheres) sphere_velocity(P, s);
+}
+
+// Contact part of the stage in one workgroup: (cell, sphere) keys sorted in
+// LDS (bitonic), contacts by the lower-index sphere, sums per sphere, velocity.
+__global__ __launch_bounds__(1024) void k_ibm_contact_stage(IbmParams P) {
+    __shared__ uint64_t key[kMaxSortKeys];
+    const int n = P.n_spheres;
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = threadIdx.x; i < m; i += 1024)
+        key[i] = i < n ? (((uint64_t)(uint32_t)P.sph[i].cell << 32) | (uint32_t)i) : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += 1024) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & k) == 0;
+                    const uint64_t a = key[i], b = key[ixj];
+                    if ((a > b) == up) { key[i] = b; key[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < n; i += 1024) sphere_contacts(P, key, i);
+    __threadfence_block();
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) sphere_contact_sum(P, key, i);
+    __threadfence_block();
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 1024) sphere_velocity(P, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -625,6 +614,137 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_interp_spread(IbmParams P, Ib
     P.pmz[i] = d[0] * dF[1] - d[1] * dF[0];
 }
 
+// One workgroup per sphere (FP32 fluids): the same interpolation and node
+// force as k_ibm_interp_spread, but the spread goes into an LDS box of
+// edge^3 nodes around the sphere (LDS float atomics), which is then added to
+// the fluid buffer once per box node; stencil nodes outside the box (nodes
+// that left the sphere's radius) fall back to global atomics. The node force
+// and moment sums of the sphere follow in the same fixed order as
+// k_ibm_reduce_nodes.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_ibm_spread_box(IbmParams P, IbmFluid<float> fl) {
+    extern __shared__ float box[];                 // [3][edge^3]
+    __shared__ double red[6][BLOCK / 64];
+    IbmSphere& c = P.sph[blockIdx.x];
+    const int edge = P.box_edge;
+    const int e3 = edge * edge * edge;
+    for (int e = threadIdx.x; e < 3 * e3; e += BLOCK) box[e] = 0.f;
+    const int half = edge / 2 - 3;                 // ceil(radius)
+    int o[3];
+    for (int a = 0; a < 3; a++) o[a] = (int)floor(c.pos[a]) - half - 2;
+    __syncthreads();
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = threadIdx.x; k < c.n_nodes; k += BLOCK) {
+        const int i = c.first_node + k;
+        const double pos[3] = {P.px[i], P.py[i], P.pz[i]};
+        const Stencil s = make_stencil(P, pos);
+        if (!s.valid) {
+            P.pdx[i] = 0; P.pdy[i] = 0; P.pdz[i] = 0;
+            P.pmx[i] = 0; P.pmy[i] = 0; P.pmz[i] = 0;
+            continue;
+        }
+        double sv[3][4];
+        for (int a = 0; a < 3; a++)
+            for (int j = 0; j < 4; j++) sv[a][j] = (j >= s.mn[a] && j <= s.mx[a]) ? peskin4(s.base[a] + j - pos[a]) : 0.0;
+        int cx[4], cy[4], cz[4], bx[4], by[4], bz[4];
+        for (int j = 0; j < 4; j++) {
+            cx[j] = stencil_coord(P, s, 0, j);
+            cy[j] = stencil_coord(P, s, 1, j);
+            cz[j] = stencil_coord(P, s, 2, j);
+            // box coordinates (periodic axes: the box may straddle the wrap)
+            int d[3] = {s.base[0] + j - o[0], s.base[1] + j - o[1], s.base[2] + j - o[2]};
+            for (int a = 0; a < 3; a++)
+                if (P.periodic[a]) {
+                    const int n = axis_n(P, a);
+                    d[a] = ((d[a] % n) + n) % n;
+                }
+            bx[j] = d[0]; by[j] = d[1]; bz[j] = d[2];
+        }
+        double rv = 0, uxv = 0, uyv = 0, uzv = 0;
+        for (int zk = s.mn[2]; zk <= s.mx[2]; zk++)
+            for (int yj = s.mn[1]; yj <= s.mx[1]; yj++) {
+                const double a1 = sv[2][zk] * sv[1][yj];
+                const size_t row = ((size_t)cz[zk] * P.ny + cy[yj]) * P.nx;
+                for (int xi = s.mn[0]; xi <= s.mx[0]; xi++) {
+                    const double a = a1 * sv[0][xi];
+                    const size_t kk = row + cx[xi];
+                    rv += (double)fl.rho[kk] * a;
+                    uxv += (double)fl.ux[kk] * a;
+                    uyv += (double)fl.uy[kk] * a;
+                    uzv += (double)fl.uz[kk] * a;
+                }
+            }
+        double d[3] = {pos[0] - c.pos[0], pos[1] - c.pos[1], pos[2] - c.pos[2]};
+        for (int a = 0; a < 3; a++) {
+            if (!P.periodic[a]) continue;
+            const double n = axis_n(P, a);
+            if (fabs(d[a]) > n / 2.0) d[a] = (d[a] < 0) ? (pos[a] + n) - c.pos[a] : (pos[a] - n) - c.pos[a];
+        }
+        double uc[3] = {0, 0, 0};
+        if (c.movable) {
+            uc[0] = c.vel[0] + (c.w[1] * d[2] - c.w[2] * d[1]);
+            uc[1] = c.vel[1] + (c.w[2] * d[0] - c.w[0] * d[2]);
+            uc[2] = c.vel[2] + (c.w[0] * d[1] - c.w[1] * d[0]);
+        }
+        const double aux = 2 * rv * P.pS[i] * P.thickness;
+        const double dF[3] = {aux * (uxv - uc[0]), aux * (uyv - uc[1]), aux * (uzv - uc[2])};
+        for (int zk = s.mn[2]; zk <= s.mx[2]; zk++)
+            for (int yj = s.mn[1]; yj <= s.mx[1]; yj++) {
+                const double a1 = sv[2][zk] * sv[1][yj];
+                for (int xi = s.mn[0]; xi <= s.mx[0]; xi++) {
+                    const double a = a1 * sv[0][xi];
+                    const float vx = (float)(-dF[0] * a), vy = (float)(-dF[1] * a), vz = (float)(-dF[2] * a);
+                    if (bx[xi] >= 0 && bx[xi] < edge && by[yj] >= 0 && by[yj] < edge && bz[zk] >= 0 && bz[zk] < edge) {
+                        const int e = (bz[zk] * edge + by[yj]) * edge + bx[xi];
+                        atomicAdd(box + e, vx);
+                        atomicAdd(box + e3 + e, vy);
+                        atomicAdd(box + 2 * e3 + e, vz);
+                    } else {
+                        const size_t kk = ((size_t)cz[zk] * P.ny + cy[yj]) * P.nx + cx[xi];
+                        unsafeAtomicAdd(fl.ax + kk, vx);
+                        unsafeAtomicAdd(fl.ay + kk, vy);
+                        unsafeAtomicAdd(fl.az + kk, vz);
+                    }
+                }
+            }
+        P.pfx[i] += dF[0]; P.pfy[i] += dF[1]; P.pfz[i] += dF[2];
+        P.pdx[i] = dF[0]; P.pdy[i] = dF[1]; P.pdz[i] = dF[2];
+        const double m3[3] = {d[1] * dF[2] - d[2] * dF[1], d[2] * dF[0] - d[0] * dF[2], d[0] * dF[1] - d[1] * dF[0]};
+        P.pmx[i] = m3[0]; P.pmy[i] = m3[1]; P.pmz[i] = m3[2];
+        acc[0] += dF[0]; acc[1] += dF[1]; acc[2] += dF[2];
+        acc[3] += m3[0]; acc[4] += m3[1]; acc[5] += m3[2];
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 6; a++) acc[a] += __shfl_down(acc[a], off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int a = 0; a < 6; a++) red[a][wave] = acc[a];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[6] = {0, 0, 0, 0, 0, 0};
+        for (int w = 0; w < BLOCK / 64; w++)
+            for (int a = 0; a < 6; a++) t[a] += red[a][w];
+        for (int a = 0; a < 3; a++) { c.f[a] += t[a]; c.M[a] += t[3 + a]; }
+    }
+    // box -> fluid buffer
+    for (int e = threadIdx.x; e < e3; e += BLOCK) {
+        const float vx = box[e], vy = box[e3 + e], vz = box[2 * e3 + e];
+        if (vx == 0.f && vy == 0.f && vz == 0.f) continue;
+        int q[3] = {o[0] + e % edge, o[1] + (e / edge) % edge, o[2] + e / (edge * edge)};
+        bool inside = true;
+        for (int a = 0; a < 3; a++) {
+            const int n = axis_n(P, a);
+            if (P.periodic[a]) q[a] = ((q[a] % n) + n) % n;
+            else if (q[a] < 0 || q[a] >= n) inside = false;
+        }
+        if (!inside) continue;
+        const size_t kk = ((size_t)q[2] * P.ny + q[1]) * P.nx + q[0];
+        unsafeAtomicAdd(fl.ax + kk, vx);
+        unsafeAtomicAdd(fl.ay + kk, vy);
+        unsafeAtomicAdd(fl.az + kk, vz);
+    }
+}
+
 // per-sphere FP64 sums of the node forces / moments (fixed-order tree)
 __global__ __launch_bounds__(kIbmBlock) void k_ibm_reduce_nodes(IbmParams P) {
     __shared__ double sh[6][kIbmBlock / 64];
@@ -654,24 +774,23 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_reduce_nodes(IbmParams P) {
 // flagged tiles: u += F_spread / (2 rho), force field += F_spread, buffer cleared
 template <typename T>
 __global__ __launch_bounds__(kIbmBlock) void k_ibm_euler_sum(IbmParams P, IbmFluid<T> fl) {
-    const uint32_t count = *P.tile_count;
-    for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
-        const uint32_t tile = P.tile_list[t];
+    const uint32_t total = *P.tile_count * (uint32_t)kTileNodes;
+    for (uint32_t e = blockIdx.x * kIbmBlock + threadIdx.x; e < total; e += gridDim.x * kIbmBlock) {
+        const uint32_t tile = P.tile_list[e / kTileNodes];
+        const int l = (int)(e % kTileNodes);
         const int tx = (int)(tile % (uint32_t)P.tiles[0]);
         const int ty = (int)((tile / (uint32_t)P.tiles[0]) % (uint32_t)P.tiles[1]);
         const int tz = (int)(tile / ((uint32_t)P.tiles[0] * (uint32_t)P.tiles[1]));
-        for (int l = threadIdx.x; l < kTileEdge * kTileEdge * kTileEdge; l += kIbmBlock) {
-            const int x = (tx << kTileShift) + (l & (kTileEdge - 1));
-            const int y = (ty << kTileShift) + ((l >> kTileShift) & (kTileEdge - 1));
-            const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
-            if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
-            const size_t k = ((size_t)z * P.ny + y) * P.nx + x;
-            const T ax = fl.ax[k], ay = fl.ay[k], az = fl.az[k];
-            const T h = (T)0.5 * ((T)1 / fl.rho[k]);
-            fl.ux[k] += ax * h; fl.uy[k] += ay * h; fl.uz[k] += az * h;
-            fl.fx[k] += ax; fl.fy[k] += ay; fl.fz[k] += az;
-            fl.ax[k] = 0; fl.ay[k] = 0; fl.az[k] = 0;
-        }
+        const int x = (tx << kTileShift) + (l & (kTileEdge - 1));
+        const int y = (ty << kTileShift) + ((l >> kTileShift) & (kTileEdge - 1));
+        const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
+        if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
+        const size_t k = ((size_t)z * P.ny + y) * P.nx + x;
+        const T ax = fl.ax[k], ay = fl.ay[k], az = fl.az[k];
+        const T h = (T)0.5 * ((T)1 / fl.rho[k]);
+        fl.ux[k] += ax * h; fl.uy[k] += ay * h; fl.uz[k] += az * h;
+        fl.fx[k] += ax; fl.fy[k] += ay; fl.fz[k] += az;
+        fl.ax[k] = 0; fl.ay[k] = 0; fl.az[k] = 0;
     }
 }
 

@@ -1,0 +1,177 @@
+// Kernels of the batched (multi-lattice) path: one persistent-grid step
+// kernel over the flat work space of all admitted lattices, the per-job
+// residual check with FP64 sums in a fixed order, and the rest-state fill.
+#include "batch.h"
+#include "kernels_impl.h"
+
+namespace vlbm {
+
+// job owning work unit w: prefix[j] <= w < prefix[j+1]
+__device__ __forceinline__ uint32_t job_of(const uint32_t* prefix, uint32_t nj, uint32_t w) {
+    uint32_t lo = 0, hi = nj;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int Q, typename S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch_step(const BatchJob<typename ComputeOf<S>::type>* __restrict__ jobs,
+                                                      const uint32_t* __restrict__ prefix,
+                                                      const uint32_t* __restrict__ n_jobs, int k) {
+    using T = typename ComputeOf<S>::type;
+    const uint32_t nj = *n_jobs;
+    const uint32_t total = nj ? prefix[nj] : 0u;
+    for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const uint32_t j = job_of(prefix, nj, w);
+        const BatchJob<T>& J = jobs[j];
+        const Params<T>& p = J.p[(J.phase + k) & 1];
+        const uint32_t idx = (w - prefix[j]) * BLOCK + threadIdx.x;
+        if (idx >= p.n) continue;
+        if (J.rheo == 0) step_node<Q, S, 0, false, false, false>(p, idx);
+        else if (J.rheo == 1) step_node<Q, S, 1, false, false, false>(p, idx);
+        else step_node<Q, S, 2, false, false, false>(p, idx);
+    }
+}
+
+template <int Q, typename S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch_check(const BatchJob<typename ComputeOf<S>::type>* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ prefix,
+                                                       const uint32_t* __restrict__ n_jobs, double* partials) {
+    using T = typename ComputeOf<S>::type;
+    __shared__ double sh[3][BLOCK / 64];
+    const uint32_t nj = *n_jobs;
+    const uint32_t total = nj ? prefix[nj] : 0u;
+    for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const uint32_t j = job_of(prefix, nj, w);
+        const BatchJob<T>& J = jobs[j];
+        if (!J.due) continue;                       // uniform per block
+        const Params<T>& p = J.p[J.phase];          // current state = source of the next step
+        const uint32_t idx = (w - prefix[j]) * BLOCK + threadIdx.x;
+        double a = 0, b = 0, c = 0;
+        if (idx < p.n) {
+            const Node<T> nd = make_node(p, idx);
+            const uint32_t word = node_word<false>(p, nd);
+            T g[Q];
+            gather<Q, S, T>(g, p, nd);
+            if (w_used(word) && w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
+            const Moments<Q, 0, T> m = moments<Q, 0, T>(g, p, (T)0, const_force(p));
+            const double ux = (double)m.ux, uy = (double)m.uy, uz = (double)m.uz;
+            const double dx = ux - (double)J.uold[0][idx], dy = uy - (double)J.uold[1][idx], dz = uz - (double)J.uold[2][idx];
+            a = sqrt(dx * dx + dy * dy + dz * dz);
+            b = sqrt(ux * ux + uy * uy + uz * uz);
+            c = (double)m.rho;
+            J.uold[0][idx] = m.ux; J.uold[1][idx] = m.uy; J.uold[2][idx] = m.uz;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_down(a, off, 64);
+            b += __shfl_down(b, off, 64);
+            c += __shfl_down(c, off, 64);
+        }
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) { sh[0][wave] = a; sh[1][wave] = b; sh[2][wave] = c; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s0 = 0, s1 = 0, s2 = 0;
+            for (int v = 0; v < BLOCK / 64; v++) { s0 += sh[0][v]; s1 += sh[1][v]; s2 += sh[2][v]; }
+            partials[3 * w + 0] = s0;
+            partials[3 * w + 1] = s1;
+            partials[3 * w + 2] = s2;
+        }
+        __syncthreads();
+    }
+}
+
+// one wave per job, fixed order over its work units
+template <typename T>
+__global__ __launch_bounds__(64) void k_batch_job_sums(const BatchJob<T>* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ prefix,
+                                                       const double* __restrict__ partials, double* sums) {
+    const uint32_t j = blockIdx.x;
+    if (!jobs[j].due) return;
+    double a = 0, b = 0, c = 0;
+    for (uint32_t w = prefix[j] + threadIdx.x; w < prefix[j + 1]; w += 64) {
+        a += partials[3 * w]; b += partials[3 * w + 1]; c += partials[3 * w + 2];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    if (threadIdx.x == 0) { sums[3 * j] = a; sums[3 * j + 1] = b; sums[3 * j + 2] = c; }
+}
+
+template <int Q, typename S>
+__global__ __launch_bounds__(kBlock) void k_batch_fill_rest(S* lat, uint32_t n, double rho0) {
+    using T = typename ComputeOf<S>::type;
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= n) return;
+#pragma unroll
+    for (int i = 0; i < Q; i++) lat[(int64_t)i * n + idx] = (S)((T)weight<Q>(i) * ((T)rho0 - (T)1));
+}
+
+template <int Q, typename S>
+static int step_s(int block, uint32_t grid, const void* jobs, const uint32_t* prefix, const uint32_t* nj, int k,
+                  hipStream_t st) {
+    using T = typename ComputeOf<S>::type;
+    const BatchJob<T>* J = (const BatchJob<T>*)jobs;
+    if (block == 128) hipLaunchKernelGGL((k_batch_step<Q, S, 128>), dim3(grid), dim3(128), 0, st, J, prefix, nj, k);
+    else hipLaunchKernelGGL((k_batch_step<Q, S, 256>), dim3(grid), dim3(256), 0, st, J, prefix, nj, k);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int Q, typename S>
+static int check_s(int block, uint32_t grid, const void* jobs, const uint32_t* prefix, const uint32_t* nj,
+                   uint32_t nj_host, double* partials, double* sums, hipStream_t st) {
+    using T = typename ComputeOf<S>::type;
+    const BatchJob<T>* J = (const BatchJob<T>*)jobs;
+    if (block == 128) hipLaunchKernelGGL((k_batch_check<Q, S, 128>), dim3(grid), dim3(128), 0, st, J, prefix, nj, partials);
+    else hipLaunchKernelGGL((k_batch_check<Q, S, 256>), dim3(grid), dim3(256), 0, st, J, prefix, nj, partials);
+    if (nj_host) hipLaunchKernelGGL((k_batch_job_sums<T>), dim3(nj_host), dim3(64), 0, st, J, prefix, partials, sums);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_batch_step(int q, int prec, int block, uint32_t grid, const void* jobs, const uint32_t* prefix,
+                      const uint32_t* n_jobs, int k, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (block != 128 && block != 256) return -1;
+    if (q == 19) {
+        if (prec == 8) return step_s<19, double>(block, grid, jobs, prefix, n_jobs, k, st);
+        if (prec == 4) return step_s<19, float>(block, grid, jobs, prefix, n_jobs, k, st);
+        return step_s<19, _Float16>(block, grid, jobs, prefix, n_jobs, k, st);
+    }
+    if (prec == 8) return step_s<27, double>(block, grid, jobs, prefix, n_jobs, k, st);
+    if (prec == 4) return step_s<27, float>(block, grid, jobs, prefix, n_jobs, k, st);
+    return step_s<27, _Float16>(block, grid, jobs, prefix, n_jobs, k, st);
+}
+
+int launch_batch_check(int q, int prec, int block, uint32_t grid, const void* jobs, const uint32_t* prefix,
+                       const uint32_t* n_jobs, uint32_t n_jobs_host, double* partials, double* sums, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (block != 128 && block != 256) return -1;
+    if (q == 19) {
+        if (prec == 8) return check_s<19, double>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
+        if (prec == 4) return check_s<19, float>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
+        return check_s<19, _Float16>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
+    }
+    if (prec == 8) return check_s<27, double>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
+    if (prec == 4) return check_s<27, float>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
+    return check_s<27, _Float16>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
+}
+
+int launch_batch_fill_rest(int q, int prec, void* lat, uint32_t n, double rho0, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid((n + kBlock - 1) / kBlock);
+#define VLBM_REST(QQ, SS) hipLaunchKernelGGL((k_batch_fill_rest<QQ, SS>), grid, dim3(kBlock), 0, st, (SS*)lat, n, rho0)
+    if (q == 19) {
+        if (prec == 8) VLBM_REST(19, double); else if (prec == 4) VLBM_REST(19, float); else VLBM_REST(19, _Float16);
+    } else {
+        if (prec == 8) VLBM_REST(27, double); else if (prec == 4) VLBM_REST(27, float); else VLBM_REST(27, _Float16);
+    }
+#undef VLBM_REST
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace vlbm

@@ -511,15 +511,13 @@ __device__ __forceinline__ void collide(T* g, const Moments<Q, RHEO, T>& m, cons
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
 
-// One time step. DIRECT: the source holds post-stream populations (first step
-// after init / load), read in place without streaming or BC. IBM: the force of
-// nodes in flagged tiles comes from the IBM force field (src/CUDA/lbm.cu:45-58
-// reads macr.f per node in IBM builds).
-template <int Q, typename S, int RHEO, bool MASK, bool DIRECT, bool IBM = false>
-__global__ __launch_bounds__(kBlock) void k_step(const Params<typename ComputeOf<S>::type> p) {
+// One node of one time step. DIRECT: the source holds post-stream
+// populations (first step after init / load), read in place without
+// streaming or BC. IBM: the force of nodes in flagged tiles comes from the
+// IBM force field (src/CUDA/lbm.cu:45-58 reads macr.f per node in IBM builds).
+template <int Q, typename S, int RHEO, bool MASK, bool DIRECT, bool IBM>
+__device__ __forceinline__ void step_node(const Params<typename ComputeOf<S>::type>& p, uint32_t idx) {
     using T = typename ComputeOf<S>::type;
-    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
-    if (idx >= p.n) return;
     const Node<T> nd = make_node(p, idx);
     const uint32_t word = node_word<MASK>(p, nd);
     if (!w_used(word)) return;
@@ -543,6 +541,14 @@ __global__ __launch_bounds__(kBlock) void k_step(const Params<typename ComputeOf
     S* dst = (S*)p.dst;
 #pragma unroll
     for (int i = 0; i < Q; i++) st_nt<S, T>(dst + (int64_t)i * p.stride + idx, g[i]);
+}
+
+// One time step over the lattice, one node per thread.
+template <int Q, typename S, int RHEO, bool MASK, bool DIRECT, bool IBM = false>
+__global__ __launch_bounds__(kBlock) void k_step(const Params<typename ComputeOf<S>::type> p) {
+    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= p.n) return;
+    step_node<Q, S, RHEO, MASK, DIRECT, IBM>(p, idx);
 }
 
 // Auxiliary pass over the current state without advancing it:
@@ -584,34 +590,33 @@ template <int Q, typename S, bool MASK>
 __global__ __launch_bounds__(kBlock) void k_ibm_macr(const Params<typename ComputeOf<S>::type> p,
                                                      const uint32_t* tile_list, const uint32_t* tile_count) {
     using T = typename ComputeOf<S>::type;
-    const uint32_t count = *tile_count;
+    const uint32_t total = *tile_count * (uint32_t)kTileNodes;
     T* ffx = (T*)p.ibm_fx;
     T* ffy = (T*)p.ibm_fy;
     T* ffz = (T*)p.ibm_fz;
-    for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
-        const uint32_t tile = tile_list[t];
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < total; e += gridDim.x * kBlock) {
+        const uint32_t tile = tile_list[e / kTileNodes];
+        const int l = (int)(e % kTileNodes);
         const int tx = (int)(tile % (uint32_t)p.tiles_x);
         const int ty = (int)((tile / (uint32_t)p.tiles_x) % (uint32_t)p.tiles_y);
         const int tz = (int)(tile / ((uint32_t)p.tiles_x * (uint32_t)p.tiles_y));
-        for (int l = threadIdx.x; l < kTileEdge * kTileEdge * kTileEdge; l += kBlock) {
-            const int x = (tx << kTileShift) + (l & (kTileEdge - 1));
-            const int y = (ty << kTileShift) + ((l >> kTileShift) & (kTileEdge - 1));
-            const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
-            if (x >= p.nx || y >= p.ny || z >= p.nz) continue;
-            const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
-            const Node<T> nd = make_node(p, idx);
-            const uint32_t word = node_word<MASK>(p, nd);
-            T g[Q];
-            if (p.direct || !w_used(word)) {
-                load_direct<Q, S, T>(g, p, nd);
-            } else {
-                gather<Q, S, T>(g, p, nd);
-                if (w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
-            }
-            const Moments<Q, 0, T> m = moments<Q, 0, T>(g, p, (T)0, const_force(p));
-            p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
-            ffx[idx] = p.fx; ffy[idx] = p.fy; ffz[idx] = p.fz;
+        const int x = (tx << kTileShift) + (l & (kTileEdge - 1));
+        const int y = (ty << kTileShift) + ((l >> kTileShift) & (kTileEdge - 1));
+        const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
+        if (x >= p.nx || y >= p.ny || z >= p.nz) continue;
+        const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+        const Node<T> nd = make_node(p, idx);
+        const uint32_t word = node_word<MASK>(p, nd);
+        T g[Q];
+        if (p.direct || !w_used(word)) {
+            load_direct<Q, S, T>(g, p, nd);
+        } else {
+            gather<Q, S, T>(g, p, nd);
+            if (w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
         }
+        const Moments<Q, 0, T> m = moments<Q, 0, T>(g, p, (T)0, const_force(p));
+        p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
+        ffx[idx] = p.fx; ffy[idx] = p.fy; ffz[idx] = p.fz;
     }
 }
 

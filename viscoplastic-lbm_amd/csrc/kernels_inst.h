@@ -40,7 +40,10 @@ struct Inst {
     static int ibm_macr_q(bool mask, const void* pv, const uint32_t* list, const uint32_t* count, uint32_t max_tiles,
                           hipStream_t st) {
         const Params<T>& p = *(const Params<T>*)pv;
-        const dim3 grid(max_tiles < 2048u ? (max_tiles ? max_tiles : 1u) : 2048u);
+        // grid-stride over the tile nodes; enough blocks to cover the tiles the
+        // spheres can touch, at most 4096
+        const uint32_t blocks = (max_tiles * (uint32_t)kTileNodes + kBlock - 1) / kBlock;
+        const dim3 grid(blocks < 4096u ? (blocks ? blocks : 1u) : 4096u);
         if (mask) hipLaunchKernelGGL((k_ibm_macr<Q, S, true>), grid, dim3(kBlock), 0, st, p, list, count);
         else hipLaunchKernelGGL((k_ibm_macr<Q, S, false>), grid, dim3(kBlock), 0, st, p, list, count);
         return hipGetLastError() == hipSuccess ? 0 : -1;

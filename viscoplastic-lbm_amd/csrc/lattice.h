@@ -126,14 +126,16 @@ struct Params {
     T duct_R, duct_r, w_out, u_out, w_in, u_in;
     // immersed boundary: nodes in 8^3 tiles flagged by the last IBM stage take
     // their force from ibm_f*, every other node the constant (fx, fy, fz)
-    const uint8_t* tile_flag;
+    const uint32_t* tile_flag;
     const T* ibm_fx; const T* ibm_fy; const T* ibm_fz;
     int tiles_x, tiles_y;
 };
 
-// Tiles of the IBM stage: 8 x 8 x 8 nodes.
-constexpr int kTileShift = 3;
+// Tiles of the IBM stage: 4 x 4 x 4 nodes (a sphere's stencil shell is a few
+// nodes thick, so small tiles keep the stage's passes near the shell).
+constexpr int kTileShift = 2;
 constexpr int kTileEdge = 1 << kTileShift;
+constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
 
 // kAuxMomentsOutsideTiles: moments with the constant force for nodes outside
 // the flagged tiles; flagged tiles keep the macroscopics of the IBM stage.

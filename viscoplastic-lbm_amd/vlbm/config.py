@@ -99,6 +99,12 @@ class SimConfig:
     inner_rotation: float = 0.0
     inner_velocity: float = 0.0
     zouhe_textbook: bool = False
+    # immersed boundary (use_ibm): vlbm_ibm_desc fields (ibmVar.h /
+    # ibmBoundaryCondition.h constants) and the spheres, each a map with
+    # center, radius and optional density, vel, w, movable, mesh
+    # ("polar" | "fibonacci"), nodes (Fibonacci node count)
+    ibm: Optional[Dict[str, Any]] = None
+    spheres: Optional[List[Dict[str, Any]]] = None
 
     def __post_init__(self):
         _validate(self)
@@ -107,6 +113,15 @@ class SimConfig:
     def id_sim(self) -> str:
         """ID_SIM as rendered by var.h.j2: sim_id zero-padded to 3 digits."""
         return "%03d" % self.sim_id
+
+    def to_ibm_desc(self) -> _capi.VlbmIbmDesc:
+        """IBM descriptor: library defaults, then the ibm map; capacities sized
+        for the listed spheres."""
+        d = _capi.default_ibm_desc(**(self.ibm or {}))
+        spheres = self.spheres or []
+        if not self.ibm or "max_spheres" not in self.ibm:
+            d.max_spheres = max(d.max_spheres, len(spheres))
+        return d
 
     def to_desc(self) -> _capi.VlbmDesc:
         d = _capi.default_desc()
@@ -190,7 +205,15 @@ def _validate(c: SimConfig) -> None:
     need(c.builder in BUILDERS, f"unknown builder {c.builder!r}")
     need(isinstance(c.path_save, str), "path_save must be a string")
     need(c.n_gpus == 1, "this library runs one lattice per GPU: n_gpus must be 1")
-    need(not c.use_ibm, "use_ibm: the immersed-boundary stage is not available in this build")
+    if c.ibm is not None:
+        need(isinstance(c.ibm, dict), "ibm must be a map of IBM descriptor fields")
+        known = {f for f, _ in _capi.VlbmIbmDesc._fields_}
+        bad = set(c.ibm) - known
+        need(not bad, f"unknown ibm keys: {sorted(bad)}")
+    for sp in c.spheres or []:
+        need(isinstance(sp, dict) and "center" in sp and "radius" in sp,
+             "each sphere needs center and radius")
+        need(sp.get("mesh", "polar") in ("polar", "fibonacci"), "sphere mesh must be 'polar' or 'fibonacci'")
     need(c.NX >= 1 and c.NY >= 1 and c.NZ >= 1, "NX, NY, NZ must be >= 1")
 
 

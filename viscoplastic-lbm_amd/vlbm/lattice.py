@@ -48,7 +48,16 @@ class Lattice:
 
     @classmethod
     def from_config(cls, cfg) -> "Lattice":
-        return cls(cfg.to_desc())
+        lat = cls(cfg.to_desc())
+        if cfg.use_ibm:
+            lat.ibm_enable(cfg.to_ibm_desc())
+            for sp in cfg.spheres or []:
+                lat.ibm_add_sphere(sp["center"], vel=sp.get("vel", (0, 0, 0)), w=sp.get("w", (0, 0, 0)),
+                                   radius=sp["radius"], density=sp.get("density", 2.0),
+                                   movable=sp.get("movable", True),
+                                   mesh=_capi.MESH_FIBONACCI if sp.get("mesh") == "fibonacci" else _capi.MESH_POLAR,
+                                   n_fibonacci=int(sp.get("nodes", 0)))
+        return lat
 
     # -- lifetime -------------------------------------------------------------
     def close(self) -> None:
