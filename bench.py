@@ -78,11 +78,22 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run only a plain device copy of BYTES (counter calibration) and print its time")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the lattice step has no "
               "multi-GPU path in this build", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import ctypes as C
+        from vlbm._capi import check, lib
+        ms = C.c_double(0)
+        check(lib.vlbm_copy_probe(args.copy_calibration, 10, C.byref(ms)))
+        print(json.dumps({"copy_bytes": args.copy_calibration, "ms": ms.value,
+                          "gbs": 2 * args.copy_calibration / (ms.value * 1e-3) / 1e9}), flush=True)
+        return
 
     # torch is not imported: its wheel bundles its own HIP runtime, and two HIP
     # runtimes in one process do not share the device. All work runs on the

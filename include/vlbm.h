@@ -162,6 +162,11 @@ int vlbm_sync(vlbm_h h);
 /* Device bytes held by the handle. */
 int vlbm_device_bytes(vlbm_h h, size_t* bytes);
 
+/* Average time of a plain device-to-device copy of `bytes` (16-byte loads and
+ * stores), the calibration of the HBM traffic counters. No reference
+ * counterpart (measurement helper). */
+int vlbm_copy_probe(size_t bytes, int iters, double* ms);
+
 /* ------------------------------------------------------------------------
  * Immersed boundary stage for rigid spheres (src/CUDA/IBM/). When enabled,
  * every vlbm_step runs the LBM step and then the stage of

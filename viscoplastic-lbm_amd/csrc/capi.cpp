@@ -559,12 +559,13 @@ int vlbm_ibm_time_steps(vlbm_h h, int n_steps, double* ms_lbm, double* ms_ibm) {
 
 int vlbm_compute_macroscopics(vlbm_h h) {
     if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
+    // with IBM the listed tiles take the macroscopics of the last stage
+    if (h->ibm) return ibm_fill_macr(h);
     const vlbm_desc& d = h->d;
     return with_params(h, [&](void* pv, auto& p) {
         p.src = h->lat[h->cur];
         p.dst = h->lat[1 - h->cur];
-        // with IBM the flagged tiles keep the macroscopics of the last stage
-        p.aux_mode = h->ibm ? kAuxMomentsOutsideTiles : kAuxMoments;
+        p.aux_mode = kAuxMoments;
         p.direct = h->direct ? 1 : 0;
         return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream), "moments kernel");
     });
@@ -690,6 +691,34 @@ int vlbm_sync(vlbm_h h) {
     if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
     HIP_TRY(hipStreamSynchronize(h->stream));
     return VLBM_OK;
+}
+
+int vlbm_copy_probe(size_t bytes, int iters, double* ms) {
+    if (!ms || iters < 1 || bytes < 16) return fail(VLBM_ERR_INVALID, "bytes >= 16, iters >= 1 and an output are required");
+    bytes = bytes / 16 * 16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = VLBM_OK;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) rc = fail(VLBM_ERR_ALLOC, "copy buffers");
+    if (!rc && (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                hipEventCreate(&e1) != hipSuccess))
+        rc = fail(VLBM_ERR_DEVICE, "stream / events");
+    if (!rc && hipMemsetAsync(a, 0, bytes, st) != hipSuccess) rc = fail(VLBM_ERR_DEVICE, "hipMemset");
+    if (!rc && launch_copy_probe(a, b, bytes, st)) rc = check_launch(-1, "copy kernel");   // warm
+    if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = fail(VLBM_ERR_DEVICE, "event");
+    for (int k = 0; k < iters && !rc; k++)
+        if (launch_copy_probe(a, b, bytes, st)) rc = check_launch(-1, "copy kernel");
+    if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)) rc = fail(VLBM_ERR_DEVICE, "event");
+    float t = 0;
+    if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = fail(VLBM_ERR_DEVICE, "elapsed time");
+    *ms = rc ? 0.0 : t / iters;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int vlbm_device_bytes(vlbm_h h, size_t* bytes) {

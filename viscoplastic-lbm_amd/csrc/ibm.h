@@ -70,14 +70,29 @@ struct IbmParams {
     int* n_contacts;
     int* error;
     int box_edge;                          // LDS spreading box edge (0: global atomics)
+    // interpolation work: chunks of at most kChunkNodes nodes of one sphere;
+    // the chunks of sphere s are [chunk0[s], chunk0[s + 1])
+    int n_chunks;
+    int* chunk_sphere;
+    int* chunk_first;                      // first node of the chunk
+    int* chunk_count;
+    int* chunk0;                           // [n_spheres + 1]
+    double* chunk_sums;                    // [n_chunks][6]: force, moment of the chunk's nodes
 };
 
+constexpr int kChunkNodes = 64;
+
+// Fluid-side fields of the stage on the listed tiles. The velocity the
+// reference carries (u plus the auxiliary F / 2 rho of every iteration so
+// far) is u + (f - f0) / (2 rho) here: the force field f starts the stage at
+// the constant force f0 and gains exactly the spread forces, so only f is
+// updated while forces spread.
 template <typename T>
 struct IbmFluid {
-    T *rho, *ux, *uy, *uz;     // macroscopics the stage interpolates
-    T *fx, *fy, *fz;           // force field read by the next step in flagged tiles
-    T *ax, *ay, *az;           // force spread in the current iteration
+    T *rho, *ux, *uy, *uz;     // rho, u of the post-step populations with the constant force
+    T *fx, *fy, *fz;           // force field (read by the next step in listed tiles)
     T f0x, f0y, f0z;           // constant force
+    T *out_rho, *out_ux, *out_uy, *out_uz;   // saved macroscopics of the handle
 };
 
 // Launchers (kernels_ibm.hip and the per-precision kernel files).
@@ -91,6 +106,7 @@ enum IbmStagePart : int {
     kIbmContacts = 1,     // cell sort, contacts, force and torque sums, velocity update
     kIbmIteration = 2,    // interpolation / spreading, sphere sums, velocity update, Euler sum
     kIbmMove = 3,         // sphere and node movement
+    kIbmOutput = 4,       // saved macroscopics on the listed tiles (not part of a stage)
 };
 
 // Edge of the LDS box that collects one sphere's spread force (FP32 fluids):

@@ -23,7 +23,7 @@ struct IbmState {
     IbmParams P;
     double r_max = 0;
     void* force[3] = {nullptr, nullptr, nullptr};
-    void* spread[3] = {nullptr, nullptr, nullptr};
+    void* macr[4] = {nullptr, nullptr, nullptr, nullptr};   // rho, u of the stage (constant force)
     std::vector<void*> allocs;
 };
 
@@ -150,10 +150,10 @@ template <typename T>
 IbmFluid<T> fluid_of(const vlbm_ctx* h) {
     const IbmState& s = *h->ibm;
     IbmFluid<T> fl;
-    fl.rho = (T*)h->macr[0]; fl.ux = (T*)h->macr[1]; fl.uy = (T*)h->macr[2]; fl.uz = (T*)h->macr[3];
+    fl.rho = (T*)s.macr[0]; fl.ux = (T*)s.macr[1]; fl.uy = (T*)s.macr[2]; fl.uz = (T*)s.macr[3];
     fl.fx = (T*)s.force[0]; fl.fy = (T*)s.force[1]; fl.fz = (T*)s.force[2];
-    fl.ax = (T*)s.spread[0]; fl.ay = (T*)s.spread[1]; fl.az = (T*)s.spread[2];
     fl.f0x = (T)h->d.fx; fl.f0y = (T)h->d.fy; fl.f0z = (T)h->d.fz;
+    fl.out_rho = (T*)h->macr[0]; fl.out_ux = (T*)h->macr[1]; fl.out_uy = (T*)h->macr[2]; fl.out_uz = (T*)h->macr[3];
     return fl;
 }
 
@@ -233,8 +233,26 @@ int add_particle(vlbm_ctx* h, const double* state, double radius, double density
         rc = sync(h);   // host buffers of this block stay alive until the copies are done
         if (rc) return rc;
     }
-    rc = sync(h);
-    if (rc) return rc;
+    // interpolation chunks of this sphere
+    {
+        std::vector<int> cs, cf, cc;
+        for (int k = 0; k < n; k += kChunkNodes) {
+            cs.push_back(sid);
+            cf.push_back(c.first_node + k);
+            cc.push_back(std::min(kChunkNodes, n - k));
+        }
+        const int nc = (int)cs.size();
+        const int end = s.P.n_chunks + nc;
+        if (nc) {
+            HIP_TRY(hipMemcpyAsync(s.P.chunk_sphere + s.P.n_chunks, cs.data(), nc * 4, hipMemcpyHostToDevice, h->stream));
+            HIP_TRY(hipMemcpyAsync(s.P.chunk_first + s.P.n_chunks, cf.data(), nc * 4, hipMemcpyHostToDevice, h->stream));
+            HIP_TRY(hipMemcpyAsync(s.P.chunk_count + s.P.n_chunks, cc.data(), nc * 4, hipMemcpyHostToDevice, h->stream));
+        }
+        HIP_TRY(hipMemcpyAsync(s.P.chunk0 + sid + 1, &end, 4, hipMemcpyHostToDevice, h->stream));
+        rc = sync(h);
+        if (rc) return rc;
+        s.P.n_chunks = end;
+    }
     s.P.n_spheres++;
     s.P.n_nodes += n;
     s.r_max = std::max(s.r_max, radius);
@@ -267,9 +285,11 @@ int ibm_stage(vlbm_ctx* h) {
     const vlbm_desc& d = h->d;
     const uint32_t max_tiles = s.P.n_tiles;
     rc = with_params(h, [&](void* pv, auto& p) {
+        using T = typename std::remove_reference<decltype(*p.m_rho)>::type;
         p.src = h->lat[h->cur];
         p.dst = h->lat[1 - h->cur];
         p.direct = h->direct ? 1 : 0;
+        p.m_rho = (T*)s.macr[0]; p.m_ux = (T*)s.macr[1]; p.m_uy = (T*)s.macr[2]; p.m_uz = (T*)s.macr[3];
         return check_launch(launch_ibm_macr(d.q, h->sbytes, h->mask_geom, pv, nullptr, s.P.tile_list, s.P.tile_count,
                                             max_tiles, h->stream),
                             "IBM macroscopics kernel");
@@ -285,13 +305,15 @@ int ibm_stage(vlbm_ctx* h) {
 // that the saved arrays hold the whole lattice after IBM steps.
 int ibm_fill_macr(vlbm_ctx* h) {
     const vlbm_desc& d = h->d;
-    return with_params(h, [&](void* pv, auto& p) {
+    int rc = with_params(h, [&](void* pv, auto& p) {
         p.src = h->lat[h->cur];
         p.dst = h->lat[1 - h->cur];
         p.aux_mode = kAuxMomentsOutsideTiles;
         p.direct = h->direct ? 1 : 0;
         return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream), "moments kernel");
     });
+    if (rc) return rc;
+    return stage_part(h, kIbmOutput);
 }
 
 int ibm_reset_tiles(vlbm_ctx* h) {
@@ -372,21 +394,23 @@ int vlbm_ibm_enable(vlbm_h h, const vlbm_ibm_desc* d) {
     A((void**)&P.tile_count, 4);
     A((void**)&P.contacts, ms * kMaxContacts * sizeof(IbmContact));
     A((void**)&P.n_contacts, ms * 4);
+    const size_t mc = mn / kChunkNodes + ms + 1;
+    A((void**)&P.chunk_sphere, mc * 4);
+    A((void**)&P.chunk_first, mc * 4);
+    A((void**)&P.chunk_count, mc * 4);
+    A((void**)&P.chunk0, (ms + 1) * 4);
+    A((void**)&P.chunk_sums, mc * 6 * sizeof(double));
     A((void**)&P.error, 4);
-    for (int a = 0; a < 3; a++) {
-        A(&s.force[a], h->n * h->tbytes);
-        A(&s.spread[a], h->n * h->tbytes);
-    }
+    for (int a = 0; a < 3; a++) A(&s.force[a], h->n * h->tbytes);
+    for (int a = 0; a < 4; a++) A(&s.macr[a], h->n * h->tbytes);
     if (rc) { ibm_release(h); return rc; }
     hipError_t e = hipSuccess;
-    for (int a = 0; a < 3 && e == hipSuccess; a++) {
-        e = hipMemsetAsync(s.spread[a], 0, h->n * h->tbytes, h->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(s.force[a], 0, h->n * h->tbytes, h->stream);
-    }
+    for (int a = 0; a < 3 && e == hipSuccess; a++) e = hipMemsetAsync(s.force[a], 0, h->n * h->tbytes, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(P.tile_flag, 0, (size_t)P.n_tiles * 4, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(P.tile_count, 0, 4, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(P.error, 0, 4, h->stream);
     if (e == hipSuccess) e = hipMemsetAsync(P.n_contacts, 0, ms * 4, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(P.chunk0, 0, (ms + 1) * 4, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
         ibm_release(h);

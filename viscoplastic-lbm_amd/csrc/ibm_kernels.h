@@ -11,10 +11,10 @@
 //             sphere, force / torque sums per sphere in a fixed order, walls,
 //             velocity update
 //   iteration (x max_iterations) one workgroup per sphere: interpolation and
-//             force of each node, spread into an LDS box around the sphere,
-//             the box added to the fluid's spread-force buffer, FP64 sums of
-//             the node forces / moments (fixed-order tree); velocity update;
-//             Euler sum u += F/(2 rho), force field += F on the listed tiles
+//             force of each node, FP64 sums of the node forces / moments
+//             (fixed-order tree); velocity update; spreading of the node
+//             forces straight into u (F / 2 rho) and the force field, through
+//             an LDS box around each sphere for FP32 fluids
 //   move      sphere and node movement
 // Everything outside the flagged tiles is left alone: the reference's full-
 // lattice passes (gpuUpdateMacrIBM, gpuEulerSumIBMAuxsReset over the bordered
@@ -140,10 +140,30 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_mark(IbmParams P) {
     }
     for (int c = 0; c < nt[2]; c++)
         for (int b = 0; b < nt[1]; b++)
-            for (int a = 0; a < nt[0]; a++) {
-                const uint32_t t = (uint32_t)(tl[0][a] + P.tiles[0] * (tl[1][b] + P.tiles[1] * tl[2][c]));
-                if (atomicExch(P.tile_flag + t, 1u) == 0u) P.tile_list[atomicAdd(P.tile_count, 1u)] = t;
-            }
+            for (int a = 0; a < nt[0]; a++)
+                P.tile_flag[tl[0][a] + P.tiles[0] * (tl[1][b] + P.tiles[1] * tl[2][c])] = 1u;
+}
+
+// flagged tiles -> list: a block-wide scan of 1024 flags, one counter atomic
+// per block (list order differs from run to run; every use of the list is
+// per node, so results do not)
+__global__ __launch_bounds__(1024) void k_ibm_list_tiles(IbmParams P) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t base;
+    const uint32_t t = blockIdx.x * 1024 + threadIdx.x;
+    const bool on = t < P.n_tiles && P.tile_flag[t] != 0u;
+    const uint64_t mask = __ballot(on);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < 16; w++) { const uint32_t v = wsum[w]; wsum[w] = tot; tot += v; }
+        base = tot ? atomicAdd(P.tile_count, tot) : 0u;
+    }
+    __syncthreads();
+    if (on) P.tile_list[base + wsum[wave] + before] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -544,136 +564,134 @@ __global__ __launch_bounds__(1024) void k_ibm_contact_stage(IbmParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// interpolation / spreading of one iteration (ibm.cu:194-542)
+// one iteration (ibm.cu:194-542 and the Euler sum, particleEulerNodesUpdate.cu:287-330)
+//
+// The reference's single kernel interpolates, computes each node's force and
+// spreads it into auxiliary fields that a later full-lattice pass adds to u
+// and f. Here the iteration is two kernels: (1) interpolation and node force
+// for every node of every sphere, (2) spreading of those forces straight into
+// u (as F / 2 rho) and the force field. All interpolation of an iteration
+// reads u before any spreading of that iteration, as in the reference.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kIbmBlock) void k_ibm_interp_spread(IbmParams P, IbmFluid<T> fl) {
-    const uint32_t i = blockIdx.x * kIbmBlock + threadIdx.x;
-    if (i >= (uint32_t)P.n_nodes) return;
-    const double pos[3] = {P.px[i], P.py[i], P.pz[i]};
-    const Stencil s = make_stencil(P, pos);
-    if (!s.valid) {
-        P.pdx[i] = 0; P.pdy[i] = 0; P.pdz[i] = 0;
-        P.pmx[i] = 0; P.pmy[i] = 0; P.pmz[i] = 0;
-        return;
-    }
+
+// Stencil weights and wrapped coordinates of a node.
+struct NodeStencil {
+    Stencil s;
     double sv[3][4];
-    for (int a = 0; a < 3; a++)
-        for (int j = 0; j < 4; j++) sv[a][j] = (j >= s.mn[a] && j <= s.mx[a]) ? peskin4(s.base[a] + j - pos[a]) : 0.0;
     int cx[4], cy[4], cz[4];
+};
+
+__device__ __forceinline__ bool node_stencil(const IbmParams& P, const double pos[3], NodeStencil& ns) {
+    ns.s = make_stencil(P, pos);
+    if (!ns.s.valid) return false;
+    for (int a = 0; a < 3; a++)
+        for (int j = 0; j < 4; j++)
+            ns.sv[a][j] = (j >= ns.s.mn[a] && j <= ns.s.mx[a]) ? peskin4(ns.s.base[a] + j - pos[a]) : 0.0;
     for (int j = 0; j < 4; j++) {
-        cx[j] = stencil_coord(P, s, 0, j);
-        cy[j] = stencil_coord(P, s, 1, j);
-        cz[j] = stencil_coord(P, s, 2, j);
+        ns.cx[j] = stencil_coord(P, ns.s, 0, j);
+        ns.cy[j] = stencil_coord(P, ns.s, 1, j);
+        ns.cz[j] = stencil_coord(P, ns.s, 2, j);
     }
-    double rv = 0, uxv = 0, uyv = 0, uzv = 0;
-    for (int zk = s.mn[2]; zk <= s.mx[2]; zk++)
-        for (int yj = s.mn[1]; yj <= s.mx[1]; yj++) {
-            const double a1 = sv[2][zk] * sv[1][yj];
-            const size_t row = ((size_t)cz[zk] * P.ny + cy[yj]) * P.nx;
-            for (int xi = s.mn[0]; xi <= s.mx[0]; xi++) {
-                const double a = a1 * sv[0][xi];
-                const size_t k = row + cx[xi];
-                rv += (double)fl.rho[k] * a;
-                uxv += (double)fl.ux[k] * a;
-                uyv += (double)fl.uy[k] * a;
-                uzv += (double)fl.uz[k] * a;
-            }
-        }
-    const IbmSphere& c = P.sph[P.pid[i]];
-    double d[3] = {pos[0] - c.pos[0], pos[1] - c.pos[1], pos[2] - c.pos[2]};
-    for (int a = 0; a < 3; a++) {
-        if (!P.periodic[a]) continue;
-        const double n = axis_n(P, a);
-        if (fabs(d[a]) > n / 2.0) d[a] = (d[a] < 0) ? (pos[a] + n) - c.pos[a] : (pos[a] - n) - c.pos[a];
-    }
-    double uc[3] = {0, 0, 0};
-    if (c.movable) {
-        uc[0] = c.vel[0] + (c.w[1] * d[2] - c.w[2] * d[1]);
-        uc[1] = c.vel[1] + (c.w[2] * d[0] - c.w[0] * d[2]);
-        uc[2] = c.vel[2] + (c.w[0] * d[1] - c.w[1] * d[0]);
-    }
-    const double aux = 2 * rv * P.pS[i] * P.thickness;
-    const double dF[3] = {aux * (uxv - uc[0]), aux * (uyv - uc[1]), aux * (uzv - uc[2])};
-    for (int zk = s.mn[2]; zk <= s.mx[2]; zk++)
-        for (int yj = s.mn[1]; yj <= s.mx[1]; yj++) {
-            const double a1 = sv[2][zk] * sv[1][yj];
-            const size_t row = ((size_t)cz[zk] * P.ny + cy[yj]) * P.nx;
-            for (int xi = s.mn[0]; xi <= s.mx[0]; xi++) {
-                const double a = a1 * sv[0][xi];
-                const size_t k = row + cx[xi];
-                unsafeAtomicAdd(fl.ax + k, (T)(-dF[0] * a));
-                unsafeAtomicAdd(fl.ay + k, (T)(-dF[1] * a));
-                unsafeAtomicAdd(fl.az + k, (T)(-dF[2] * a));
-            }
-        }
-    P.pfx[i] += dF[0]; P.pfy[i] += dF[1]; P.pfz[i] += dF[2];
-    P.pdx[i] = dF[0]; P.pdy[i] = dF[1]; P.pdz[i] = dF[2];
-    P.pmx[i] = d[1] * dF[2] - d[2] * dF[1];
-    P.pmy[i] = d[2] * dF[0] - d[0] * dF[2];
-    P.pmz[i] = d[0] * dF[1] - d[1] * dF[0];
+    return true;
 }
 
-// One workgroup per sphere (FP32 fluids): the same interpolation and node
-// force as k_ibm_interp_spread, but the spread goes into an LDS box of
-// edge^3 nodes around the sphere (LDS float atomics), which is then added to
-// the fluid buffer once per box node; stencil nodes outside the box (nodes
-// that left the sphere's radius) fall back to global atomics. The node force
-// and moment sums of the sphere follow in the same fixed order as
-// k_ibm_reduce_nodes.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_ibm_spread_box(IbmParams P, IbmFluid<float> fl) {
-    extern __shared__ float box[];                 // [3][edge^3]
-    __shared__ double red[6][BLOCK / 64];
-    IbmSphere& c = P.sph[blockIdx.x];
-    const int edge = P.box_edge;
-    const int e3 = edge * edge * edge;
-    for (int e = threadIdx.x; e < 3 * e3; e += BLOCK) box[e] = 0.f;
-    const int half = edge / 2 - 3;                 // ceil(radius)
-    int o[3];
-    for (int a = 0; a < 3; a++) o[a] = (int)floor(c.pos[a]) - half - 2;
-    __syncthreads();
+// One wave per Lagrangian node, one lane per point of its 4^3 stencil
+// (lane = x + 4 y + 16 z), so the loads and stores of a wave fall on 16 rows
+// of 4 neighbouring nodes. Lanes 0..11 evaluate the 12 kernel weights and
+// the others read theirs with shuffles.
+constexpr int kInterpBlock = 256;
+constexpr int kSphereBlock = 1024;
+
+struct PointOf {
+    int xi, yj, zk;
+    bool in;          // the point survives the wall clipping
+    double a;         // its weight (phi_z phi_y) phi_x, 0 when clipped
+};
+
+__device__ __forceinline__ PointOf stencil_point(const Stencil& st, const double pos[3], int lane) {
+    PointOf pt;
+    pt.xi = lane & 3;
+    pt.yj = (lane >> 2) & 3;
+    pt.zk = lane >> 4;
+    double wl = 0.0;
+    if (lane < 12) {
+        const int a = lane >> 2, j = lane & 3;
+        if (j >= st.mn[a] && j <= st.mx[a]) wl = peskin4(st.base[a] + j - pos[a]);
+    }
+    const double wx = __shfl(wl, pt.xi, 64), wy = __shfl(wl, 4 + pt.yj, 64), wz = __shfl(wl, 8 + pt.zk, 64);
+    pt.in = pt.xi >= st.mn[0] && pt.xi <= st.mx[0] && pt.yj >= st.mn[1] && pt.yj <= st.mx[1] &&
+            pt.zk >= st.mn[2] && pt.zk <= st.mx[2];
+    pt.a = pt.in ? (wz * wy) * wx : 0.0;
+    return pt;
+}
+
+// (1) one workgroup per chunk of up to 64 nodes of one sphere, 16 lanes per
+// node (one per (z, y) row of the stencil): interpolated rho and u
+// (u + (f - f0) / 2 rho, the velocity after the iterations so far) over the
+// lane's row, the 16 row sums combined by a butterfly; the node's force
+// dF = 2 rho dA thickness (u - u_node) and its moment about the centre; the
+// chunk's force and moment sums in a fixed order (strided per-thread partial
+// sums, wave64 tree), added to the sphere by k_ibm_add_chunks.
+constexpr int kRowLanes = 16;
+
+template <typename T>
+__global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFluid<T> fl) {
+    __shared__ double red[6][kInterpBlock / 64];
+    const int chunk = blockIdx.x;
+    const IbmSphere& c = P.sph[P.chunk_sphere[chunk]];
+    const int first = P.chunk_first[chunk];
+    const int r = threadIdx.x & (kRowLanes - 1);
+    const int zk = r >> 2, yj = r & 3;
+    const int items = P.chunk_count[chunk] * kRowLanes;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = threadIdx.x; k < c.n_nodes; k += BLOCK) {
-        const int i = c.first_node + k;
-        const double pos[3] = {P.px[i], P.py[i], P.pz[i]};
-        const Stencil s = make_stencil(P, pos);
-        if (!s.valid) {
+    for (int b0 = 0; b0 < items; b0 += kInterpBlock) {      // uniform trip count for the shuffles
+        const int k = b0 + (int)threadIdx.x;
+        const bool act = k < items;
+        const int i = first + k / kRowLanes;
+        double pv[4] = {0, 0, 0, 0};
+        double pos[3] = {0, 0, 0};
+        Stencil st;
+        st.valid = false;
+        if (act) {
+            pos[0] = P.px[i]; pos[1] = P.py[i]; pos[2] = P.pz[i];
+            st = make_stencil(P, pos);
+            if (st.valid && zk >= st.mn[2] && zk <= st.mx[2] && yj >= st.mn[1] && yj <= st.mx[1]) {
+                const double a1 = peskin4(st.base[2] + zk - pos[2]) * peskin4(st.base[1] + yj - pos[1]);
+                const size_t row = ((size_t)stencil_coord(P, st, 2, zk) * P.ny + stencil_coord(P, st, 1, yj)) * P.nx;
+                // all four x points at once (weight 0 and an in-range node for
+                // points the wall clips), so the loads issue together
+                double wx[4];
+                size_t kx[4];
+#pragma unroll
+                for (int xi = 0; xi < 4; xi++) {
+                    const bool in = xi >= st.mn[0] && xi <= st.mx[0];
+                    wx[xi] = in ? a1 * peskin4(st.base[0] + xi - pos[0]) : 0.0;
+                    kx[xi] = row + stencil_coord(P, st, 0, in ? xi : st.mn[0]);
+                }
+                T tr[4], tx[4], ty[4], tz[4], fx[4], fy[4], fz[4];
+#pragma unroll
+                for (int xi = 0; xi < 4; xi++) {
+                    tr[xi] = fl.rho[kx[xi]]; tx[xi] = fl.ux[kx[xi]]; ty[xi] = fl.uy[kx[xi]]; tz[xi] = fl.uz[kx[xi]];
+                    fx[xi] = fl.fx[kx[xi]]; fy[xi] = fl.fy[kx[xi]]; fz[xi] = fl.fz[kx[xi]];
+                }
+#pragma unroll
+                for (int xi = 0; xi < 4; xi++) {
+                    const T h = (T)0.5 / tr[xi];
+                    pv[0] += (double)tr[xi] * wx[xi];
+                    pv[1] += (double)(tx[xi] + (fx[xi] - fl.f0x) * h) * wx[xi];
+                    pv[2] += (double)(ty[xi] + (fy[xi] - fl.f0y) * h) * wx[xi];
+                    pv[3] += (double)(tz[xi] + (fz[xi] - fl.f0z) * h) * wx[xi];
+                }
+            }
+        }
+        for (int off = 1; off < kRowLanes; off <<= 1)
+            for (int q = 0; q < 4; q++) pv[q] += __shfl_xor(pv[q], off, 64);
+        if (!act || r != 0) continue;
+        if (!st.valid) {
             P.pdx[i] = 0; P.pdy[i] = 0; P.pdz[i] = 0;
             P.pmx[i] = 0; P.pmy[i] = 0; P.pmz[i] = 0;
             continue;
         }
-        double sv[3][4];
-        for (int a = 0; a < 3; a++)
-            for (int j = 0; j < 4; j++) sv[a][j] = (j >= s.mn[a] && j <= s.mx[a]) ? peskin4(s.base[a] + j - pos[a]) : 0.0;
-        int cx[4], cy[4], cz[4], bx[4], by[4], bz[4];
-        for (int j = 0; j < 4; j++) {
-            cx[j] = stencil_coord(P, s, 0, j);
-            cy[j] = stencil_coord(P, s, 1, j);
-            cz[j] = stencil_coord(P, s, 2, j);
-            // box coordinates (periodic axes: the box may straddle the wrap)
-            int d[3] = {s.base[0] + j - o[0], s.base[1] + j - o[1], s.base[2] + j - o[2]};
-            for (int a = 0; a < 3; a++)
-                if (P.periodic[a]) {
-                    const int n = axis_n(P, a);
-                    d[a] = ((d[a] % n) + n) % n;
-                }
-            bx[j] = d[0]; by[j] = d[1]; bz[j] = d[2];
-        }
-        double rv = 0, uxv = 0, uyv = 0, uzv = 0;
-        for (int zk = s.mn[2]; zk <= s.mx[2]; zk++)
-            for (int yj = s.mn[1]; yj <= s.mx[1]; yj++) {
-                const double a1 = sv[2][zk] * sv[1][yj];
-                const size_t row = ((size_t)cz[zk] * P.ny + cy[yj]) * P.nx;
-                for (int xi = s.mn[0]; xi <= s.mx[0]; xi++) {
-                    const double a = a1 * sv[0][xi];
-                    const size_t kk = row + cx[xi];
-                    rv += (double)fl.rho[kk] * a;
-                    uxv += (double)fl.ux[kk] * a;
-                    uyv += (double)fl.uy[kk] * a;
-                    uzv += (double)fl.uz[kk] * a;
-                }
-            }
         double d[3] = {pos[0] - c.pos[0], pos[1] - c.pos[1], pos[2] - c.pos[2]};
         for (int a = 0; a < 3; a++) {
             if (!P.periodic[a]) continue;
@@ -686,30 +704,11 @@ __global__ __launch_bounds__(BLOCK) void k_ibm_spread_box(IbmParams P, IbmFluid<
             uc[1] = c.vel[1] + (c.w[2] * d[0] - c.w[0] * d[2]);
             uc[2] = c.vel[2] + (c.w[0] * d[1] - c.w[1] * d[0]);
         }
-        const double aux = 2 * rv * P.pS[i] * P.thickness;
-        const double dF[3] = {aux * (uxv - uc[0]), aux * (uyv - uc[1]), aux * (uzv - uc[2])};
-        for (int zk = s.mn[2]; zk <= s.mx[2]; zk++)
-            for (int yj = s.mn[1]; yj <= s.mx[1]; yj++) {
-                const double a1 = sv[2][zk] * sv[1][yj];
-                for (int xi = s.mn[0]; xi <= s.mx[0]; xi++) {
-                    const double a = a1 * sv[0][xi];
-                    const float vx = (float)(-dF[0] * a), vy = (float)(-dF[1] * a), vz = (float)(-dF[2] * a);
-                    if (bx[xi] >= 0 && bx[xi] < edge && by[yj] >= 0 && by[yj] < edge && bz[zk] >= 0 && bz[zk] < edge) {
-                        const int e = (bz[zk] * edge + by[yj]) * edge + bx[xi];
-                        atomicAdd(box + e, vx);
-                        atomicAdd(box + e3 + e, vy);
-                        atomicAdd(box + 2 * e3 + e, vz);
-                    } else {
-                        const size_t kk = ((size_t)cz[zk] * P.ny + cy[yj]) * P.nx + cx[xi];
-                        unsafeAtomicAdd(fl.ax + kk, vx);
-                        unsafeAtomicAdd(fl.ay + kk, vy);
-                        unsafeAtomicAdd(fl.az + kk, vz);
-                    }
-                }
-            }
+        const double aux = 2 * pv[0] * P.pS[i] * P.thickness;
+        const double dF[3] = {aux * (pv[1] - uc[0]), aux * (pv[2] - uc[1]), aux * (pv[3] - uc[2])};
+        const double m3[3] = {d[1] * dF[2] - d[2] * dF[1], d[2] * dF[0] - d[0] * dF[2], d[0] * dF[1] - d[1] * dF[0]};
         P.pfx[i] += dF[0]; P.pfy[i] += dF[1]; P.pfz[i] += dF[2];
         P.pdx[i] = dF[0]; P.pdy[i] = dF[1]; P.pdz[i] = dF[2];
-        const double m3[3] = {d[1] * dF[2] - d[2] * dF[1], d[2] * dF[0] - d[0] * dF[2], d[0] * dF[1] - d[1] * dF[0]};
         P.pmx[i] = m3[0]; P.pmy[i] = m3[1]; P.pmz[i] = m3[2];
         acc[0] += dF[0]; acc[1] += dF[1]; acc[2] += dF[2];
         acc[3] += m3[0]; acc[4] += m3[1]; acc[5] += m3[2];
@@ -722,12 +721,93 @@ __global__ __launch_bounds__(BLOCK) void k_ibm_spread_box(IbmParams P, IbmFluid<
     __syncthreads();
     if (threadIdx.x == 0) {
         double t[6] = {0, 0, 0, 0, 0, 0};
-        for (int w = 0; w < BLOCK / 64; w++)
+        for (int w = 0; w < kInterpBlock / 64; w++)
             for (int a = 0; a < 6; a++) t[a] += red[a][w];
-        for (int a = 0; a < 3; a++) { c.f[a] += t[a]; c.M[a] += t[3 + a]; }
+        for (int a = 0; a < 6; a++) P.chunk_sums[6 * chunk + a] = t[a];
     }
-    // box -> fluid buffer
-    for (int e = threadIdx.x; e < e3; e += BLOCK) {
+}
+
+// chunk sums -> sphere force and torque, chunks in order
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_add_chunks(IbmParams P) {
+    const int s = blockIdx.x * kIbmBlock + threadIdx.x;
+    if (s >= P.n_spheres) return;
+    IbmSphere& c = P.sph[s];
+    double t[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = P.chunk0[s]; k < P.chunk0[s + 1]; k++)
+        for (int a = 0; a < 6; a++) t[a] += P.chunk_sums[6 * k + a];
+    for (int a = 0; a < 3; a++) { c.f[a] += t[a]; c.M[a] += t[3 + a]; }
+}
+
+// (2a) spreading into the force field with global atomics, one wave per node
+// (FP64 fluids, and FP32 when the LDS box does not fit).
+template <typename T>
+__global__ __launch_bounds__(kInterpBlock) void k_ibm_spread(IbmParams P, IbmFluid<T> fl) {
+    const int lane = threadIdx.x & 63;
+    const int i = (int)((blockIdx.x * kInterpBlock + threadIdx.x) >> 6);
+    if (i >= P.n_nodes) return;
+    const double pos[3] = {P.px[i], P.py[i], P.pz[i]};
+    const Stencil st = make_stencil(P, pos);
+    if (!st.valid) return;
+    const PointOf pt = stencil_point(st, pos, lane);
+    if (!pt.in) return;
+    const size_t kk = ((size_t)stencil_coord(P, st, 2, pt.zk) * P.ny + stencil_coord(P, st, 1, pt.yj)) * P.nx +
+                      stencil_coord(P, st, 0, pt.xi);
+    unsafeAtomicAdd(fl.fx + kk, (T)(-P.pdx[i] * pt.a));
+    unsafeAtomicAdd(fl.fy + kk, (T)(-P.pdy[i] * pt.a));
+    unsafeAtomicAdd(fl.fz + kk, (T)(-P.pdz[i] * pt.a));
+}
+
+// (2b) FP32 fluids: one workgroup per sphere, 16 lanes per node (one stencil
+// row each), collects the spread force in an LDS box of edge^3 nodes around
+// the sphere (LDS float atomics) and adds the box to the force field with one
+// global atomic per box node and component. Points outside the box (nodes
+// that left the sphere's radius) go straight to global memory.
+__global__ __launch_bounds__(kSphereBlock) void k_ibm_spread_box(IbmParams P, IbmFluid<float> fl) {
+    extern __shared__ float box[];                 // [3][edge^3]
+    const IbmSphere& c = P.sph[blockIdx.x];
+    const int edge = P.box_edge;
+    const int e3 = edge * edge * edge;
+    for (int e = threadIdx.x; e < 3 * e3; e += kSphereBlock) box[e] = 0.f;
+    const int half = edge / 2 - 3;                 // ceil(radius)
+    int o[3];
+    for (int a = 0; a < 3; a++) o[a] = (int)floor(c.pos[a]) - half - 2;
+    __syncthreads();
+    const int r = threadIdx.x & (kRowLanes - 1);
+    const int zk = r >> 2, yj = r & 3;
+    const int items = c.n_nodes * kRowLanes;
+    for (int k = threadIdx.x; k < items; k += kSphereBlock) {
+        const int i = c.first_node + k / kRowLanes;
+        const double pos[3] = {P.px[i], P.py[i], P.pz[i]};
+        const Stencil st = make_stencil(P, pos);
+        if (!st.valid || zk < st.mn[2] || zk > st.mx[2] || yj < st.mn[1] || yj > st.mx[1]) continue;
+        const double a1 = peskin4(st.base[2] + zk - pos[2]) * peskin4(st.base[1] + yj - pos[1]);
+        const int cz = stencil_coord(P, st, 2, zk), cy = stencil_coord(P, st, 1, yj);
+        int bz = st.base[2] + zk - o[2], by = st.base[1] + yj - o[1];
+        if (P.periodic[2]) bz = ((bz % P.nz) + P.nz) % P.nz;
+        if (P.periodic[1]) by = ((by % P.ny) + P.ny) % P.ny;
+        const double dF[3] = {P.pdx[i], P.pdy[i], P.pdz[i]};
+#pragma unroll
+        for (int xi = 0; xi < 4; xi++) {
+            if (xi < st.mn[0] || xi > st.mx[0]) continue;
+            const double a = a1 * peskin4(st.base[0] + xi - pos[0]);
+            const float vx = (float)(-dF[0] * a), vy = (float)(-dF[1] * a), vz = (float)(-dF[2] * a);
+            int bx = st.base[0] + xi - o[0];
+            if (P.periodic[0]) bx = ((bx % P.nx) + P.nx) % P.nx;
+            if (bx >= 0 && bx < edge && by >= 0 && by < edge && bz >= 0 && bz < edge) {
+                const int e = (bz * edge + by) * edge + bx;
+                atomicAdd(box + e, vx);
+                atomicAdd(box + e3 + e, vy);
+                atomicAdd(box + 2 * e3 + e, vz);
+            } else {
+                const size_t kk = ((size_t)cz * P.ny + cy) * P.nx + stencil_coord(P, st, 0, xi);
+                unsafeAtomicAdd(fl.fx + kk, vx);
+                unsafeAtomicAdd(fl.fy + kk, vy);
+                unsafeAtomicAdd(fl.fz + kk, vz);
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < e3; e += kSphereBlock) {
         const float vx = box[e], vy = box[e3 + e], vz = box[2 * e3 + e];
         if (vx == 0.f && vy == 0.f && vz == 0.f) continue;
         int q[3] = {o[0] + e % edge, o[1] + (e / edge) % edge, o[2] + e / (edge * edge)};
@@ -739,41 +819,16 @@ __global__ __launch_bounds__(BLOCK) void k_ibm_spread_box(IbmParams P, IbmFluid<
         }
         if (!inside) continue;
         const size_t kk = ((size_t)q[2] * P.ny + q[1]) * P.nx + q[0];
-        unsafeAtomicAdd(fl.ax + kk, vx);
-        unsafeAtomicAdd(fl.ay + kk, vy);
-        unsafeAtomicAdd(fl.az + kk, vz);
+        unsafeAtomicAdd(fl.fx + kk, vx);
+        unsafeAtomicAdd(fl.fy + kk, vy);
+        unsafeAtomicAdd(fl.fz + kk, vz);
     }
 }
 
-// per-sphere FP64 sums of the node forces / moments (fixed-order tree)
-__global__ __launch_bounds__(kIbmBlock) void k_ibm_reduce_nodes(IbmParams P) {
-    __shared__ double sh[6][kIbmBlock / 64];
-    IbmSphere& c = P.sph[blockIdx.x];
-    const int first = c.first_node, n = c.n_nodes;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = threadIdx.x; k < n; k += kIbmBlock) {
-        const int i = first + k;
-        acc[0] += P.pdx[i]; acc[1] += P.pdy[i]; acc[2] += P.pdz[i];
-        acc[3] += P.pmx[i]; acc[4] += P.pmy[i]; acc[5] += P.pmz[i];
-    }
-    for (int off = 32; off > 0; off >>= 1)
-        for (int a = 0; a < 6; a++) acc[a] += __shfl_down(acc[a], off, 64);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int a = 0; a < 6; a++) sh[a][wave] = acc[a];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t[6] = {0, 0, 0, 0, 0, 0};
-        for (int w = 0; w < kIbmBlock / 64; w++)
-            for (int a = 0; a < 6; a++) t[a] += sh[a][w];
-        for (int a = 0; a < 3; a++) { c.f[a] += t[a]; c.M[a] += t[3 + a]; }
-    }
-}
-
-// gpuEulerSumIBMAuxsReset (particleEulerNodesUpdate.cu:287-330) on the
-// flagged tiles: u += F_spread / (2 rho), force field += F_spread, buffer cleared
+// saved macroscopics on the listed tiles: rho and the velocity after the
+// stage, u + (f - f0) / 2 rho (the reference's u after its Euler sums)
 template <typename T>
-__global__ __launch_bounds__(kIbmBlock) void k_ibm_euler_sum(IbmParams P, IbmFluid<T> fl) {
+__global__ __launch_bounds__(kIbmBlock) void k_ibm_output(IbmParams P, IbmFluid<T> fl) {
     const uint32_t total = *P.tile_count * (uint32_t)kTileNodes;
     for (uint32_t e = blockIdx.x * kIbmBlock + threadIdx.x; e < total; e += gridDim.x * kIbmBlock) {
         const uint32_t tile = P.tile_list[e / kTileNodes];
@@ -786,11 +841,12 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_euler_sum(IbmParams P, IbmFlu
         const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
         if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
         const size_t k = ((size_t)z * P.ny + y) * P.nx + x;
-        const T ax = fl.ax[k], ay = fl.ay[k], az = fl.az[k];
-        const T h = (T)0.5 * ((T)1 / fl.rho[k]);
-        fl.ux[k] += ax * h; fl.uy[k] += ay * h; fl.uz[k] += az * h;
-        fl.fx[k] += ax; fl.fy[k] += ay; fl.fz[k] += az;
-        fl.ax[k] = 0; fl.ay[k] = 0; fl.az[k] = 0;
+        const T r = fl.rho[k];
+        const T h = (T)0.5 / r;
+        fl.out_rho[k] = r;
+        fl.out_ux[k] = fl.ux[k] + (fl.fx[k] - fl.f0x) * h;
+        fl.out_uy[k] = fl.uy[k] + (fl.fy[k] - fl.f0y) * h;
+        fl.out_uz[k] = fl.uz[k] + (fl.fz[k] - fl.f0z) * h;
     }
 }
 

@@ -82,6 +82,20 @@ int launch_reduce(int prec, const void* rho, const void* ux, const void* uy, con
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Plain streaming copy (16-byte loads and stores), the calibration kernel of
+// the traffic counters.
+__global__ __launch_bounds__(kBlock) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                       size_t n) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) dst[i] = src[i];
+}
+
+int launch_copy_probe(const void* src, void* dst, size_t bytes, void* stream) {
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(kBlock), 0, (hipStream_t)stream, (const float4*)src,
+                       (float4*)dst, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_convert(int prec, void* buf, double* dbl, size_t count, int to_double, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     const dim3 grid((unsigned)((count + kBlock - 1) / kBlock));

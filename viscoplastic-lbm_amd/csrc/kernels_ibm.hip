@@ -7,31 +7,32 @@ namespace vlbm {
 
 static inline dim3 blocks_for(uint32_t n) { return dim3(n ? (n + kIbmBlock - 1) / kIbmBlock : 1u); }
 
+static inline dim3 wave_per_node_grid(uint32_t nodes) {
+    return dim3((uint32_t)(((uint64_t)nodes * 64 + kInterpBlock - 1) / kInterpBlock));
+}
+
 template <typename T>
 static int spread(const IbmParams& P, const IbmFluid<T>& fl, hipStream_t st) {
-    const uint32_t ns = (uint32_t)P.n_spheres, nn = (uint32_t)P.n_nodes;
-    hipLaunchKernelGGL((k_ibm_interp_spread<T>), blocks_for(nn), dim3(kIbmBlock), 0, st, P, fl);
-    hipLaunchKernelGGL(k_ibm_reduce_nodes, dim3(ns), dim3(kIbmBlock), 0, st, P);
+    hipLaunchKernelGGL((k_ibm_spread<T>), wave_per_node_grid((uint32_t)P.n_nodes), dim3(kInterpBlock), 0, st, P, fl);
     return 0;
 }
 
 template <>
 int spread<float>(const IbmParams& P, const IbmFluid<float>& fl, hipStream_t st) {
-    const uint32_t ns = (uint32_t)P.n_spheres, nn = (uint32_t)P.n_nodes;
     if (P.box_edge <= 0) {
-        hipLaunchKernelGGL((k_ibm_interp_spread<float>), blocks_for(nn), dim3(kIbmBlock), 0, st, P, fl);
-        hipLaunchKernelGGL(k_ibm_reduce_nodes, dim3(ns), dim3(kIbmBlock), 0, st, P);
+        hipLaunchKernelGGL((k_ibm_spread<float>), wave_per_node_grid((uint32_t)P.n_nodes), dim3(kInterpBlock), 0,
+                           st, P, fl);
         return 0;
     }
     const size_t lds = 3 * sizeof(float) * (size_t)P.box_edge * P.box_edge * P.box_edge;
     static size_t lds_set = 0;
     if (lds > lds_set) {
-        if (hipFuncSetAttribute((const void*)k_ibm_spread_box<512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (hipFuncSetAttribute((const void*)k_ibm_spread_box, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds) != hipSuccess)
             return -1;
         lds_set = lds;
     }
-    hipLaunchKernelGGL(k_ibm_spread_box<512>, dim3(ns), dim3(512), lds, st, P, fl);
+    hipLaunchKernelGGL(k_ibm_spread_box, dim3((uint32_t)P.n_spheres), dim3(kSphereBlock), lds, st, P, fl);
     return 0;
 }
 
@@ -39,23 +40,28 @@ template <typename T>
 static int stage_part(const IbmParams& P, const IbmFluid<T>& fl, int part, hipStream_t st) {
     const dim3 blk(kIbmBlock);
     const uint32_t ns = (uint32_t)P.n_spheres, nn = (uint32_t)P.n_nodes;
-    // grid-stride passes over the tile nodes: enough blocks for the tiles
-    // listed, at most 4096
-    const uint32_t tile_blocks = (P.n_tiles * (uint32_t)kTileNodes + kIbmBlock - 1) / kIbmBlock;
-    const dim3 tile_grid(tile_blocks < 4096u ? (tile_blocks ? tile_blocks : 1u) : 4096u);
     switch (part) {
     case kIbmBegin:
         hipLaunchKernelGGL(k_ibm_begin, blocks_for(ns > P.n_tiles ? ns : P.n_tiles), blk, 0, st, P);
         if (nn) hipLaunchKernelGGL(k_ibm_mark, blocks_for(nn), blk, 0, st, P);
+        hipLaunchKernelGGL(k_ibm_list_tiles, dim3((P.n_tiles + 1023) / 1024), dim3(1024), 0, st, P);
         break;
     case kIbmContacts:
         if (ns) hipLaunchKernelGGL(k_ibm_contact_stage, dim3(1), dim3(1024), 0, st, P);
         break;
     case kIbmIteration:
-        if (nn && spread<T>(P, fl, st)) return -1;
+        if (nn) {
+            hipLaunchKernelGGL((k_ibm_interp<T>), dim3((uint32_t)P.n_chunks), dim3(kInterpBlock), 0, st, P, fl);
+            hipLaunchKernelGGL(k_ibm_add_chunks, blocks_for(ns), blk, 0, st, P);
+        }
         if (ns) hipLaunchKernelGGL(k_ibm_velocity, blocks_for(ns), blk, 0, st, P);
-        hipLaunchKernelGGL((k_ibm_euler_sum<T>), tile_grid, blk, 0, st, P, fl);
+        if (nn && spread<T>(P, fl, st)) return -1;
         break;
+    case kIbmOutput: {
+        const uint32_t blocks = (P.n_tiles * (uint32_t)kTileNodes + kIbmBlock - 1) / kIbmBlock;
+        hipLaunchKernelGGL((k_ibm_output<T>), dim3(blocks < 4096u ? (blocks ? blocks : 1u) : 4096u), blk, 0, st, P, fl);
+        break;
+    }
     case kIbmMove:
         if (ns) hipLaunchKernelGGL(k_ibm_move, blocks_for(ns), blk, 0, st, P);
         if (nn) hipLaunchKernelGGL(k_ibm_move_nodes, blocks_for(nn), blk, 0, st, P);

@@ -150,5 +150,6 @@ int launch_reduce(int prec, const void* rho, const void* ux, const void* uy, con
 int launch_fill_feq(int q, int prec, void* state, int64_t stride, uint32_t n, const double* rho,
                     const double* ux, const double* uy, const double* uz, void* stream);
 int launch_convert(int prec, void* buf, double* dbl, size_t count, int to_double, void* stream);
+int launch_copy_probe(const void* src, void* dst, size_t bytes, void* stream);
 
 } // namespace vlbm
