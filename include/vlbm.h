@@ -149,7 +149,28 @@ int vlbm_save_macr_bin(vlbm_h h, const char* path, const char* id, unsigned step
 int vlbm_save_info(vlbm_h h, const char* path, const char* id, unsigned total_steps,
                    double mlups, double seconds);
 
-/* Steps executed since vlbm_init / vlbm_load_pops. */
+/* Writes the checkpoint folder PATH/ID/checkpoint/ in the layout of
+ * saveSimCheckpoint / operateSimCheckpoint (src/CUDA/simCheckpoint.cu:216-318):
+ * 0_curr_step.bin (int32 step), 0_pop.bin and 0_popAux.bin (Q planes of
+ * (NZ+1)*NY*NX, the post-stream state the next step collides), 0_rho.bin and
+ * 0_u/{x,y,z}.bin (NX*NY*(NZ+2B), B = 3 border planes with IBM, else 0),
+ * 0_omega.bin (non-Newtonian), and with IBM 0_f/{x,y,z}.bin, the
+ * ParticleCenter records 0_IBM_particles_centers.bin and the
+ * 0_IBM_nodes_* arrays. Values are double for FP64 lattices, float otherwise.
+ * PATH must exist; ID and checkpoint are created. Passing vlbm_steps_done
+ * makes a loaded run continue exactly; the reference's in-loop checkpoints
+ * (src/CUDA/main.cu:452-455) store the label of the step just run, one less. */
+int vlbm_save_checkpoint(vlbm_h h, const char* path, const char* id, int step);
+
+/* Loads a checkpoint folder written by vlbm_save_checkpoint or by the
+ * reference (loadSimCheckpoint, src/CUDA/simCheckpoint.cu:297-305) into a
+ * handle of the same descriptor (and, with IBM, the same spheres and node
+ * counts); file sizes are checked against the handle. The step counter
+ * becomes the stored step, which also labels the next IBM stage. */
+int vlbm_load_checkpoint(vlbm_h h, const char* path, const char* id, int* step);
+
+/* Steps executed since vlbm_init / vlbm_load_pops, or the step stored by the
+ * checkpoint that vlbm_load_checkpoint read. */
 int vlbm_steps_done(vlbm_h h, long long* steps);
 
 /* Runs n_steps and reports the device time between two HIP events recorded

@@ -1124,6 +1124,7 @@ struct PCenter {
     int partner[kMaxColl];
     double tang[kMaxColl][3];
     int last[kMaxColl];
+    double axis[3][3];   // collision.semiAxis, semiAxis2, semiAxis3 (points)
 };
 
 // Peskin 4-point kernel (src/CUDA/IBM/ibmGlobalFunctions.h:28-48)
@@ -1561,7 +1562,10 @@ struct Ibm {
         return std::fmod(v + Na, Na);
     }
 
-    // gpuParticleMovement (ibm.cu:899-993); semi-axes are not used by spheres
+    // gpuParticleMovement (ibm.cu:899-993). The semi-axis points are not used
+    // by sphere contacts; they rotate about the centre with the node rotation
+    // (including its (qi*qj - q0*qj) entry) and no guard on |w_avg| = 0, which
+    // makes them NaN after a step without rotation (ibm.cu:958-991).
     void movement() {
         const double al = d.alpha;
         for (auto& c : pc) {
@@ -1574,6 +1578,18 @@ struct Ibm {
             for (int a = 0; a < 3; a++) {
                 c.w_avg[a] = (c.w[a] * al + c.w_old[a] * (1.0 - al));
                 c.w_pos[a] += c.w_avg[a];
+            }
+            const double wn = std::sqrt(c.w_avg[0] * c.w_avg[0] + c.w_avg[1] * c.w_avg[1] + c.w_avg[2] * c.w_avg[2]);
+            const double q0 = std::cos(0.5 * wn);
+            const double qi = (c.w_avg[0] / wn) * std::sin(0.5 * wn);
+            const double qj = (c.w_avg[1] / wn) * std::sin(0.5 * wn);
+            const double qk = (c.w_avg[2] / wn) * std::sin(0.5 * wn);
+            const double t = (q0 * q0) - 0.5;
+            for (auto& ax : c.axis) {
+                const double x = ax[0] - c.pos_old[0], y = ax[1] - c.pos_old[1], z = ax[2] - c.pos_old[2];
+                ax[0] = c.pos[0] + 2 * ((t + (qi * qi)) * x + ((qi * qj) - (q0 * qk)) * y + ((qi * qk) + (q0 * qj)) * z);
+                ax[1] = c.pos[1] + 2 * (((qi * qj) + (q0 * qk)) * x + (t + (qj * qj)) * y + ((qj * qk) - (q0 * qi)) * z);
+                ax[2] = c.pos[2] + 2 * (((qi * qj) - (q0 * qj)) * x + ((qj * qk) + (q0 * qi)) * y + (t + (qk * qk)) * z);
             }
         }
     }
@@ -1869,6 +1885,8 @@ int or_ibm_add_particle(void* hv, const double* state, double radius, double den
     c.I[0] = c.I[1] = c.I[2] = Ival;
     c.movable = movable != 0;
     for (int i = 0; i < kMaxColl; i++) { c.partner[i] = -1; c.last[i] = -1; }
+    // semiAxis = (r, r, r) (particle.cpp:777-778); semiAxis2/3 unset there, 0 here
+    for (int a = 0; a < 3; a++) { c.axis[0][a] = radius; c.axis[1][a] = 0; c.axis[2][a] = 0; }
     const int id = (int)I.pc.size();
     I.pc.push_back(c);
     for (int k = 0; k < n_nodes; k++) {
@@ -1893,6 +1911,12 @@ void or_ibm_run(void* hv, int nsteps, int first_step, int nthreads) {
 void or_ibm_stage(void* hv, int step) { ((Handle*)hv)->ibm->stage(step); }
 void or_ibm_collisions(void* hv, int step) { ((Handle*)hv)->ibm->collisions(step); }
 void or_ibm_update_velocity(void* hv) { ((Handle*)hv)->ibm->update_velocity(); }
+
+// sphere and node movement of the current velocities (the end of a stage)
+void or_ibm_move(void* hv) {
+    ((Handle*)hv)->ibm->movement();
+    ((Handle*)hv)->ibm->node_movement();
+}
 
 int or_ibm_count(void* hv, int* n_nodes) {
     Handle* h = (Handle*)hv;
@@ -1928,6 +1952,14 @@ void or_ibm_set_particles(void* hv, const double* in) {
         double* dst2[] = {c.pos_old, c.vel_old, c.w_old, c.w_avg, c.w_pos, c.f_old, c.M_old, c.dP, c.dL};
         for (auto s : dst2) for (int a = 0; a < 3; a++) s[a] = o[k++];
     }
+}
+
+// semi-axis points, 9 doubles per particle (semiAxis, semiAxis2, semiAxis3)
+void or_ibm_get_axes(void* hv, double* out) {
+    Ibm& I = *((Handle*)hv)->ibm;
+    for (size_t p = 0; p < I.pc.size(); p++)
+        for (int k = 0; k < 3; k++)
+            for (int a = 0; a < 3; a++) out[9 * p + 3 * k + a] = I.pc[p].axis[k][a];
 }
 
 void or_ibm_get_collisions(void* hv, int* partner, double* tang, int* last) {

@@ -107,6 +107,8 @@ def lib():
         L.or_ibm_count.restype = C.c_int
         L.or_ibm_count.argtypes = [C.c_void_p, ip]
         L.or_ibm_get_particles.argtypes = [C.c_void_p, dp]
+        L.or_ibm_get_axes.argtypes = [C.c_void_p, dp]
+        L.or_ibm_move.argtypes = [C.c_void_p]
         L.or_ibm_set_particles.argtypes = [C.c_void_p, dp]
         L.or_ibm_get_collisions.argtypes = [C.c_void_p, ip, dp, ip]
         L.or_ibm_get_nodes.argtypes = [C.c_void_p, dp, dp, dp]
@@ -229,6 +231,9 @@ class Oracle:
     def ibm_collisions(self, step):
         lib().or_ibm_collisions(self.h, int(step))
 
+    def ibm_move(self):
+        lib().or_ibm_move(self.h)
+
     def ibm_update_velocity(self):
         lib().or_ibm_update_velocity(self.h)
 
@@ -242,6 +247,14 @@ class Oracle:
         out = np.zeros((np_, PARTICLE_RECORD))
         if np_:
             lib().or_ibm_get_particles(self.h, _p(out))
+        return out
+
+    def ibm_axes(self) -> np.ndarray:
+        """Semi-axis points (semiAxis, semiAxis2, semiAxis3) per particle, (n, 3, 3)."""
+        np_, _ = self.ibm_counts()
+        out = np.zeros((np_, 3, 3))
+        if np_:
+            lib().or_ibm_get_axes(self.h, _p(out))
         return out
 
     def ibm_set_particles(self, rec):

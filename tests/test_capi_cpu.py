@@ -181,3 +181,31 @@ def test_sweep_job_generator_and_bingham_number():
     # Newtonian limit: the profile is the Poiseuille parabola
     s = np.linspace(-4, 4, 9)
     assert np.allclose(serve.bingham_profile(s, 4.0, 1e-5, 0.1, 0.0), 1e-5 / 0.2 * (16 - s * s))
+
+
+def test_reference_particle_center_layout():
+    """The checkpoint's ParticleCenter record: 8 dfloat3, 2 dfloat4, 4 dfloat3,
+    dfloat6, 2 dfloat3, 4 dfloat, bool, then CollisionData (int, 3 dfloat3,
+    int[13], dfloat3[13], int[13]); offsets by hand for both dfloat types."""
+    from checkpoint_layout import particle_center_dtype
+    for t, s, size in ((np.float64, 8, 992), (np.float32, 4, 544)):
+        d = particle_center_dtype(t)
+        assert d.itemsize == size
+        assert d.fields["q_pos"][1] == 24 * s
+        assert d.fields["I"][1] == 44 * s
+        assert d.fields["S"][1] == 56 * s
+        assert d.fields["movable"][1] == 60 * s
+        assert d.fields["collision"][1] == 61 * s    # bool padded to the next dfloat boundary
+        c = d.fields["collision"][0]
+        assert c.fields["semiAxis"][1] == s          # int padded to a dfloat
+        assert c.fields["partner"][1] == s + 9 * s
+        assert c.fields["tang"][1] == -(-(10 * s + 52) // s) * s
+        assert c.fields["last"][1] == c.fields["tang"][1] + 39 * s
+
+
+def test_checkpoint_keys_in_config():
+    from helpers import base_cfg
+    cfg = base_cfg(checkpoint_save=500, load_checkpoint="RUNS")
+    assert cfg.checkpoint_save == 500 and cfg.load_checkpoint == "RUNS"
+    with pytest.raises(ValueError):
+        base_cfg(checkpoint_save=-1)

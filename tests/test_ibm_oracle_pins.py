@@ -407,3 +407,53 @@ def test_translation_and_rotation_about_z():
     R = Rotation.from_rotvec(w).as_matrix()
     expect = (c + v) + (R @ (xyz2 - c).T).T
     assert np.allclose(pos.T, expect, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# semi-axis points of the particle record (ibm.cu:958-991)
+# ---------------------------------------------------------------------------
+def quat(w):
+    """Unit quaternion of the rotation by |w| about w: (q0, qi, qj, qk)."""
+    n = np.linalg.norm(w)
+    return np.concatenate([[math.cos(n / 2)], np.asarray(w) / n * math.sin(n / 2)])
+
+
+@pytest.mark.parametrize("w", [(0.0, 0.0, 0.02), (0.011, 0.013, 0.0), (0.004, -0.009, 0.017)])
+def test_semi_axes_rotate_with_the_reference_matrix(w):
+    """The three semi-axis points move to pos + M (p - pos_old) where M is the
+    rotation matrix of the quaternion of w_avg (scipy) except for entry
+    (z, x), which the reference computes as 2 (qi qj - q0 qj) instead of
+    2 (qi qk - q0 qj). A rotation about z leaves that entry exact."""
+    w = np.array(w)
+    c = np.array([16.0, 15.5, 16.2])
+    v = np.array([0.002, 0.001, -0.003])
+    r = 3.0
+    o = dem_oracle(dict(periodic_y=1, ref_float_helpers=0, max_iterations=0, alpha=1.0), nx=32, ny=32,
+                   nz=32, builder="taylorGreenVortex")
+    o.ibm_add_particle(c, v, w, r, 2.0, True, np.zeros((0, 3)), np.zeros(0))
+    ax0 = o.ibm_axes()[0]
+    assert np.array_equal(ax0, [[r, r, r], [0, 0, 0], [0, 0, 0]])   # particle.cpp:777-778
+    o.ibm_move()
+    q0, qi, qj, qk = quat(w)
+    M = Rotation.from_rotvec(w).as_matrix()
+    M[2, 0] += 2 * qi * qj - 2 * qi * qk
+    expect = (c + v) + (M @ (ax0 - c).T).T
+    assert np.allclose(o.ibm_axes()[0], expect, rtol=0, atol=1e-12)
+    if w[0] == 0 and w[1] == 0:
+        R = Rotation.from_rotvec(w).as_matrix()
+        assert np.allclose(expect, (c + v) + (R @ (ax0 - c).T).T, atol=1e-14)
+
+
+def test_semi_axes_become_nan_without_rotation():
+    """No guard on |w_avg| = 0: q = (1, 0/0, 0/0, 0/0) makes every semi-axis
+    coordinate NaN, while positions and nodes (which take the translation
+    branch) stay finite."""
+    c = np.array([12.0, 12.0, 12.0])
+    o = dem_oracle(dict(periodic_y=1, ref_float_helpers=0, max_iterations=0, alpha=1.0), nx=24, ny=24,
+                   nz=24, builder="taylorGreenVortex")
+    one_sphere(o, c, vel=(0.001, 0, 0), r=3.0)
+    o.ibm_move()
+    assert np.isnan(o.ibm_axes()).all()
+    pos, _, _ = o.ibm_nodes()
+    assert np.isfinite(pos).all()
+    assert np.allclose(o.ibm_particles()[0, F["pos"]], c + [0.001, 0, 0], atol=1e-15)

@@ -38,6 +38,7 @@ SOURCES = [
     "ibm_capi.cpp",
     "kernels_batch.hip",
     "batch_capi.cpp",
+    "checkpoint.cpp",
     "capi.cpp",
 ]
 

@@ -27,6 +27,9 @@ struct IbmSphere {
     int partner[kMaxColl];
     int last[kMaxColl];
     double tang[kMaxColl][3];
+    // collision.semiAxis{,2,3} points of the reference's ParticleCenter: moved
+    // with the sphere, unused by sphere contacts, kept for the checkpoint file
+    double axis[3][3];
 };
 
 // Sphere-sphere contact of sphere i (the lower index) with sphere j.

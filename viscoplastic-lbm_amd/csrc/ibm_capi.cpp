@@ -12,22 +12,10 @@
 
 #include "ctx.h"
 #include "ibm.h"
+#include "ibm_state.h"
 
 using namespace vlbm;
 using namespace vlbm::detail;
-
-namespace vlbm {
-
-struct IbmState {
-    vlbm_ibm_desc d;
-    IbmParams P;
-    double r_max = 0;
-    void* force[3] = {nullptr, nullptr, nullptr};
-    void* macr[4] = {nullptr, nullptr, nullptr, nullptr};   // rho, u of the stage (constant force)
-    std::vector<void*> allocs;
-};
-
-}  // namespace vlbm
 
 namespace {
 
@@ -214,6 +202,9 @@ int add_particle(vlbm_ctx* h, const double* state, double radius, double density
     c.first_node = s.P.n_nodes;
     c.n_nodes = n;
     for (int k = 0; k < kMaxColl; k++) { c.partner[k] = -1; c.last[k] = -1; }
+    // collision.semiAxis = (r, r, r) as a point; semiAxis2/3 are not set for
+    // spheres there (particle.cpp:777-778) and start at 0 here
+    for (int a = 0; a < 3; a++) c.axis[0][a] = radius;
     const int sid = s.P.n_spheres;
     HIP_TRY(hipMemcpyAsync(s.P.sph + sid, &c, sizeof(c), hipMemcpyHostToDevice, h->stream));
     if (n > 0) {

@@ -868,6 +868,20 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_move(IbmParams P) {
         c.w_avg[a] = (c.w[a] * al + c.w_old[a] * (1.0 - al));
         c.w_pos[a] += c.w_avg[a];
     }
+    // semi-axis points rotate about the centre with no guard on |w_avg| = 0
+    // (ibm.cu:958-991): a step without rotation leaves them NaN, as there
+    const double wn = sqrt(c.w_avg[0] * c.w_avg[0] + c.w_avg[1] * c.w_avg[1] + c.w_avg[2] * c.w_avg[2]);
+    const double q0 = cos(0.5 * wn);
+    const double qi = (c.w_avg[0] / wn) * sin(0.5 * wn);
+    const double qj = (c.w_avg[1] / wn) * sin(0.5 * wn);
+    const double qk = (c.w_avg[2] / wn) * sin(0.5 * wn);
+    const double t = (q0 * q0) - 0.5;
+    for (int k = 0; k < 3; k++) {
+        const double v[3] = {c.axis[k][0] - c.pos_old[0], c.axis[k][1] - c.pos_old[1], c.axis[k][2] - c.pos_old[2]};
+        c.axis[k][0] = c.pos[0] + 2 * ((t + (qi * qi)) * v[0] + ((qi * qj) - (q0 * qk)) * v[1] + ((qi * qk) + (q0 * qj)) * v[2]);
+        c.axis[k][1] = c.pos[1] + 2 * (((qi * qj) + (q0 * qk)) * v[0] + (t + (qj * qj)) * v[1] + ((qj * qk) - (q0 * qi)) * v[2]);
+        c.axis[k][2] = c.pos[2] + 2 * (((qi * qj) - (q0 * qj)) * v[0] + ((qj * qk) + (q0 * qi)) * v[1] + (t + (qk * qk)) * v[2]);
+    }
 }
 
 // the rotation keeps the reference's (qi*qj) - (q0*qj) entry (ibm.cu:1174)

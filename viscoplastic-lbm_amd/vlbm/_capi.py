@@ -95,6 +95,8 @@ def _load() -> C.CDLL:
         "vlbm_save_macr_bin": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_uint]),
         "vlbm_save_info": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_uint, C.c_double, C.c_double]),
         "vlbm_steps_done": (C.c_int, [C.c_void_p, P(C.c_longlong)]),
+        "vlbm_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]),
+        "vlbm_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, P(C.c_int)]),
         "vlbm_time_steps": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "vlbm_sync": (C.c_int, [C.c_void_p]),
         "vlbm_device_bytes": (C.c_int, [C.c_void_p, P(C.c_size_t)]),

@@ -21,6 +21,9 @@ file); here they are optional extra keys with the reference defaults:
     duct_outer_radius, duct_inner_radius, outer_rotation, outer_velocity,
     inner_rotation, inner_velocity        # duct builders
     zouhe_textbook: false                 # textbook Zou-He instead of the reference lines
+    checkpoint_save: 0                    # CHECKPOINT_SAVE interval (var.h:69-71)
+    load_checkpoint: null                 # folder to resume from: LOAD_CHECKPOINT with
+                                          # SIMULATION_FOLDER_LOAD_CHECKPOINT (var.h:77-84)
 """
 from __future__ import annotations
 
@@ -99,6 +102,8 @@ class SimConfig:
     inner_rotation: float = 0.0
     inner_velocity: float = 0.0
     zouhe_textbook: bool = False
+    checkpoint_save: int = 0
+    load_checkpoint: Optional[str] = None
     # immersed boundary (use_ibm): vlbm_ibm_desc fields (ibmVar.h /
     # ibmBoundaryCondition.h constants) and the spheres, each a map with
     # center, radius and optional density, vel, w, movable, mesh
@@ -174,7 +179,8 @@ def _validate(c: SimConfig) -> None:
             raise ValueError(msg)
 
     # numbers are coerced from strings as pydantic does (YAML 1.1 reads 1e-5 as a string)
-    for name in ("NX", "NY", "NZ", "sim_id", "steps", "macr_save", "data_report", "ini_step", "n_gpus"):
+    for name in ("NX", "NY", "NZ", "sim_id", "steps", "macr_save", "data_report", "ini_step", "n_gpus",
+                 "checkpoint_save"):
         v = getattr(c, name)
         if isinstance(v, str):
             try:
@@ -199,6 +205,8 @@ def _validate(c: SimConfig) -> None:
         setattr(c, name, float(v))
     for name in ("use_ibm", "data_stop", "data_save", "pop_save", "zouhe_textbook"):
         need(isinstance(getattr(c, name), bool), f"{name} must be a boolean")
+    need(c.checkpoint_save >= 0, "checkpoint_save must be >= 0")
+    need(c.load_checkpoint is None or isinstance(c.load_checkpoint, str), "load_checkpoint must be a folder name")
     need(c.precision in PRECISIONS, "precision must be 'single' or 'double' (or 'half')")
     need(c.vel_set in ("D3Q19", "D3Q27"), "vel_set must be 'D3Q19' or 'D3Q27'")
     need(c.non_newtonian_model in MODELS, "non_newtonian_model must be null, BINGHAM or POWERLAW")

@@ -154,6 +154,18 @@ class Lattice:
     def save_macr_bin(self, path: str, sim_id: str, step: int) -> None:
         check(lib.vlbm_save_macr_bin(self._h, path.encode(), sim_id.encode(), int(step)))
 
+    def save_checkpoint(self, path: str, sim_id: str, step: Optional[int] = None) -> None:
+        """PATH/ID/checkpoint/ in the reference's layout; step defaults to the
+        steps done, which makes a loaded run continue exactly."""
+        s = self.steps_done if step is None else int(step)
+        check(lib.vlbm_save_checkpoint(self._h, path.encode(), sim_id.encode(), s))
+
+    def load_checkpoint(self, path: str, sim_id: str) -> int:
+        """Loads PATH/ID/checkpoint/; returns the stored step."""
+        s = C.c_int(0)
+        check(lib.vlbm_load_checkpoint(self._h, path.encode(), sim_id.encode(), C.byref(s)))
+        return s.value
+
     def save_info(self, path: str, sim_id: str, total_steps: int, mlups: float, seconds: float) -> None:
         check(lib.vlbm_save_info(self._h, path.encode(), sim_id.encode(), int(total_steps),
                                  float(mlups), float(seconds)))

@@ -10,6 +10,11 @@ Per step s (counted from ini_step):
     (main.cu:460-465), report and the stop test residual < resid_max or
     mean density < 0 when data_stop is set (main.cu:468-485,
     src/CUDA/treatData.cpp:451-459);
+  * checkpoint_save: every checkpoint_save steps the checkpoint folder is
+    written after the step, holding the label of the step just run
+    (main.cu:452-456), and once more after the loop (main.cu:546-548);
+    load_checkpoint: the run starts from that folder's checkpoint and its
+    stored step instead of the equilibrium initialisation (main.cu:199-203);
   * after the loop the saved macroscopics are written once more, labelled with
     the loop counter, so the final file holds the moments computed during the
     last step (main.cu:523-527), then ID_info.txt.
@@ -40,7 +45,7 @@ def _next_sync(step: int, cfg: SimConfig) -> int:
     """First step >= step that saves or reports (or the last step)."""
     last = cfg.steps - 1
     best = last
-    for every in (cfg.macr_save, cfg.data_report):
+    for every in (cfg.macr_save, cfg.data_report, cfg.checkpoint_save):
         if every and every > 0:
             aux = step - cfg.ini_step
             k = max(1, math.ceil(aux / every)) * every + cfg.ini_step
@@ -51,13 +56,17 @@ def _next_sync(step: int, cfg: SimConfig) -> int:
 def run(cfg: SimConfig, write_files: bool = True, log: Callable[[str], None] = print,
         lattice: Optional[Lattice] = None) -> RunResult:
     lat = lattice or Lattice.from_config(cfg)
-    lat.init()
     folder = os.path.join(cfg.path_save, cfg.id_sim)
     if write_files:
         os.makedirs(folder, exist_ok=True)
     result = RunResult(steps=0, seconds=0.0, mlups=0.0)
     nodes = cfg.NX * cfg.NY * cfg.NZ
-    step = cfg.ini_step
+    if cfg.load_checkpoint:
+        step = lat.load_checkpoint(cfg.load_checkpoint, cfg.id_sim)
+    else:
+        lat.init()
+        step = cfg.ini_step
+    first_step = step
     lat.sync()
     t_start = time.perf_counter()
     t_sync, last_sync_step = t_start, step
@@ -66,6 +75,7 @@ def run(cfg: SimConfig, write_files: bool = True, log: Callable[[str], None] = p
         aux = target - cfg.ini_step
         save = bool(cfg.macr_save) and aux != 0 and aux % cfg.macr_save == 0
         rep = bool(cfg.data_report) and aux != 0 and aux % cfg.data_report == 0
+        ckpt = bool(cfg.checkpoint_save) and aux != 0 and aux % cfg.checkpoint_save == 0
         last = target + 1 >= cfg.steps
         lat.step(target - step + 1, save_last=save or rep or last)
         step = target
@@ -89,14 +99,18 @@ def run(cfg: SimConfig, write_files: bool = True, log: Callable[[str], None] = p
                     result.stopped = "residual" if residual < cfg.resid_max else "negative density"
                     log(f"Stopping because of LBM ({result.stopped})")
                     break
+        if ckpt and write_files:
+            lat.save_checkpoint(cfg.path_save, cfg.id_sim, step)
         step += 1
     lat.sync()
     elapsed = time.perf_counter() - t_start
-    total = step - cfg.ini_step
+    total = step - first_step
     result.steps = total
     result.seconds = elapsed
     result.mlups = nodes * total / 1e6 / max(elapsed, 1e-12)
     if write_files:
+        if cfg.checkpoint_save:
+            lat.save_checkpoint(cfg.path_save, cfg.id_sim, step)
         lat.save_macr_bin(cfg.path_save, cfg.id_sim, step)
         lat.save_info(cfg.path_save, cfg.id_sim, total, result.mlups, elapsed)
     if lattice is None:
