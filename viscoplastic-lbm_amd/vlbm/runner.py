@@ -15,6 +15,11 @@ Per step s (counted from ini_step):
     (main.cu:452-456), and once more after the loop (main.cu:546-548);
     load_checkpoint: the run starts from that folder's checkpoint and its
     stored step instead of the equilibrium initialisation (main.cu:199-203);
+  * data_save: every report also updates the x-z plane averages of u_z per y
+    without resetting them first, avg <- (avg + sum_xz u_z) / (NX NZ)
+    (treatData, src/CUDA/treatData.cpp:57-78), and writes them to
+    ID_avgUz<step>.csv (saveTreatData, treatData.cpp:476-497); once more after
+    the loop, labelled with the loop counter (main.cu:553-558);
   * after the loop the saved macroscopics are written once more, labelled with
     the loop counter, so the final file holds the moments computed during the
     last step (main.cu:523-527), then ID_info.txt.
@@ -53,6 +58,20 @@ def _next_sync(step: int, cfg: SimConfig) -> int:
     return max(step, best)
 
 
+def var_filename(path: str, sim_id: str, var: str, step: int, ext: str) -> str:
+    """getVarFilename (src/CUDA/lbmReport.cpp:55-83): the step zero-padded to
+    six digits (step 0 gets six zeros before it)."""
+    zeros = 6 if step == 0 else sum(1 for k in range(1, 7) if step * 10 ** k < 1000000)
+    return os.path.join(path, sim_id, f"{sim_id}_{var}" + "0" * zeros + str(step) + ext)
+
+
+def save_avg_uz(path: str, sim_id: str, step: int, avg_uz) -> None:
+    with open(var_filename(path, sim_id, "avgUz", step, ".csv"), "w") as f:
+        f.write("y\tavg uz\n")
+        for y, v in enumerate(avg_uz):
+            f.write("%d\t%.6e\n" % (y, v))
+
+
 def run(cfg: SimConfig, write_files: bool = True, log: Callable[[str], None] = print,
         lattice: Optional[Lattice] = None) -> RunResult:
     lat = lattice or Lattice.from_config(cfg)
@@ -67,6 +86,7 @@ def run(cfg: SimConfig, write_files: bool = True, log: Callable[[str], None] = p
         lat.init()
         step = cfg.ini_step
     first_step = step
+    avg_uz = [0.0] * cfg.NY
     lat.sync()
     t_start = time.perf_counter()
     t_sync, last_sync_step = t_start, step
@@ -89,6 +109,13 @@ def run(cfg: SimConfig, write_files: bool = True, log: Callable[[str], None] = p
                 lat.save_macr_bin(cfg.path_save, cfg.id_sim, step)
             if rep:
                 entry = {"step": step, "residual": residual, "avg_rho": mean_rho, "mlups": mlups}
+                if cfg.data_save:
+                    uz = lat.macroscopics()["uz"]
+                    plane = uz.sum(axis=(0, 2))
+                    avg_uz = [(a + p) / (cfg.NX * cfg.NZ) for a, p in zip(avg_uz, plane)]
+                    entry["avg_uz_mid"] = avg_uz[cfg.NY // 2]
+                    if write_files:
+                        save_avg_uz(cfg.path_save, cfg.id_sim, step, avg_uz)
                 result.reports.append(entry)
                 log(f"step {step}: residual {residual:.4e}, avg density {mean_rho:.4e}, MLUPS {mlups:.1f}")
                 if not math.isfinite(mean_rho):
@@ -113,6 +140,8 @@ def run(cfg: SimConfig, write_files: bool = True, log: Callable[[str], None] = p
             lat.save_checkpoint(cfg.path_save, cfg.id_sim, step)
         lat.save_macr_bin(cfg.path_save, cfg.id_sim, step)
         lat.save_info(cfg.path_save, cfg.id_sim, total, result.mlups, elapsed)
+        if cfg.data_report and cfg.data_save:
+            save_avg_uz(cfg.path_save, cfg.id_sim, step, avg_uz)
     if lattice is None:
         lat.close()
     return result
