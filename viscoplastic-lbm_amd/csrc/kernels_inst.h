@@ -14,9 +14,16 @@ struct Inst {
     static int step_q(bool mask, bool direct, bool ibm, const void* pv, hipStream_t st) {
         const Params<T>& p = *(const Params<T>*)pv;
         const dim3 grid = grid_of(p.n), blk(kBlock);
-        if (ibm && !direct) {
-            if (mask) hipLaunchKernelGGL((k_step<Q, S, RHEO, true, false, true>), grid, blk, 0, st, p);
-            else hipLaunchKernelGGL((k_step<Q, S, RHEO, false, false, true>), grid, blk, 0, st, p);
+        if (ibm) {
+            // the IBM force field applies to the first step after a checkpoint
+            // load too (post-stream state read in place)
+            if (mask) {
+                if (direct) hipLaunchKernelGGL((k_step<Q, S, RHEO, true, true, true>), grid, blk, 0, st, p);
+                else hipLaunchKernelGGL((k_step<Q, S, RHEO, true, false, true>), grid, blk, 0, st, p);
+            } else {
+                if (direct) hipLaunchKernelGGL((k_step<Q, S, RHEO, false, true, true>), grid, blk, 0, st, p);
+                else hipLaunchKernelGGL((k_step<Q, S, RHEO, false, false, true>), grid, blk, 0, st, p);
+            }
         } else if (mask) {
             if (direct) hipLaunchKernelGGL((k_step<Q, S, RHEO, true, true>), grid, blk, 0, st, p);
             else hipLaunchKernelGGL((k_step<Q, S, RHEO, true, false>), grid, blk, 0, st, p);
