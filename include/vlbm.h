@@ -37,6 +37,11 @@ extern "C" {
 #define VLBM_BINGHAM 1
 #define VLBM_POWERLAW 2
 
+/* Collision schemes: regularized (src/CUDA/lbm.cu:263-461, the compiled
+ * default) or BGK (src/CUDA/collisionSchemes/recursiveCollisionForceNoReg.txt). */
+#define VLBM_COLLISION_REGULARIZED 0
+#define VLBM_COLLISION_BGK 1
+
 /* boundary-condition builders (src/CUDA/boundaryConditionsBuilders/) */
 #define VLBM_BUILDER_TAYLOR_GREEN 0          /* taylorGreenVortex.cu: fully periodic */
 #define VLBM_BUILDER_PLATES_BB_Y 1           /* parallelPlatesBounceBack_Y.cu */
@@ -76,6 +81,8 @@ typedef struct vlbm_desc {
     double duct_outer_radius, duct_inner_radius;      /* OUTER_RADIUS, INNER_RADIUS (var.h:141-160) */
     double outer_rotation, outer_velocity, inner_rotation, inner_velocity;
     int zouhe_textbook;      /* 0: reproduce the reference's Zou-He lines; 1: textbook Zou-He */
+    int collision;           /* VLBM_COLLISION_REGULARIZED or VLBM_COLLISION_BGK */
+    double les_const;        /* LES_MODEL's Implicit_const (lbm.cu:226-242); 0 = no LES, Newtonian only */
 } vlbm_desc;
 
 typedef struct vlbm_ctx* vlbm_h;

@@ -103,6 +103,8 @@ struct or_desc {
     double ux_bc[8], uy_bc[8], uz_bc[8], rho_bc[8];   // src/CUDA/var.h:114-117
     double duct_outer_radius, duct_inner_radius;      // interpolated-BB builders
     double outer_rotation, outer_velocity, inner_rotation, inner_velocity;
+    int collision;         // 0 regularized (lbm.cu:263-461), 1 BGK (recursiveCollisionForceNoReg.txt)
+    double les_const;      // LES_MODEL Implicit_const (lbm.cu:226-242); 0 = no LES
 };
 
 } // extern "C"
@@ -546,7 +548,7 @@ struct Lattice {
         const R mYZt2 = (sYZ - rhoVar * uyVar * uzVar) * 2;
 
         R omegaVar = (R)(1.0 / (R)d.tau);
-        if (d.rheo != 0) {
+        if (d.rheo != 0 || d.les_const != 0) {
             // stress magnitude (src/CUDA/lbm.cu:192-208)
             const R uFxxd2 = uxVar * fxVar, uFyyd2 = uyVar * fyVar, uFzzd2 = uzVar * fzVar;
             const R uFxyd2 = (uxVar * fyVar + uyVar * fxVar) / 2;
@@ -560,15 +562,27 @@ struct Lattice {
                      (mYZt2 / 2 + uFyzd2) * (mYZt2 / 2 + uFyzd2))));
             if (d.rheo == 1) {
                 omegaVar = calc_omega((R)(1 / (3 * (R)d.eta_p + 0.5)), stressMag);
-            } else {
+            } else if (d.rheo == 2) {
                 omegaVar = calc_omega(*omega_io, stressMag);
                 *omega_io = omegaVar;
+            }
+            if (d.les_const != 0) {
+                // LES_MODEL (lbm.cu:226-242): implicit Smagorinsky eddy relaxation
+                const R TAU = (R)d.tau;
+                const R tau_t = 0.5 * std::sqrt(TAU * TAU + (R)d.les_const * stressMag) - 0.5 * TAU;
+                omegaVar = 1.0 / (TAU + tau_t);
             }
         }
         const R t_omegaVar = 1 - omegaVar;
         const R tt_omegaVar = 1 - 0.5 * omegaVar;
         m_out[0] = rhoVar; m_out[1] = uxVar; m_out[2] = uyVar; m_out[3] = uzVar; m_out[4] = omegaVar;
 
+        const bool bgk = d.collision == 1;
+        if (bgk) {
+            // BGK: fNode *= T_OMEGA, then += OMEGA rho w terms
+            // (recursiveCollisionForceNoReg.txt:38-104)
+            for (int i = 0; i < q; i++) f[i] *= t_omegaVar;
+        } else {
         // regularized non-equilibrium (src/CUDA/lbm.cu:263-325)
         terms[0] = -mXX / 3 - mYY / 3 - mZZ / 3;
         terms[1] = terms[0] + (-fxVar_D3 + mXX);
@@ -614,8 +628,10 @@ struct Lattice {
             f[26] = mt * (terms[10] + (-fxVar_D3 + mXX - mXYt2 - mXZt2));
         }
         for (int i = 0; i < q; i++) f[i] *= t_omegaVar;
+        }
 
-        // equilibrium (src/CUDA/lbm.cu:341-401)
+        // equilibrium (src/CUDA/lbm.cu:341-401); BGK scales it by omega
+        const R eqs = bgk ? omegaVar : R(1);
         terms[0] = p1_muu15;
         terms[1] = terms[0] + (ux3 + ux3ux3d2);
         terms[2] = terms[0] + (-ux3 + ux3ux3d2);
@@ -629,13 +645,13 @@ struct Lattice {
             terms[9] = terms[3] + (uz3 + uy3uz3 + uz3uz3d2);
             terms[10] = terms[4] + (-uz3 + uy3uz3 + uz3uz3d2);
         }
-        mt = rhoW0;
+        mt = eqs * rhoW0;
         f[0] += mt * terms[0];
-        mt = rhoW1;
+        mt = eqs * rhoW1;
         f[1] += mt * terms[1]; f[2] += mt * terms[2]; f[3] += mt * terms[3]; f[4] += mt * terms[4];
         f[5] += mt * (terms[0] + (uz3 + uz3uz3d2));
         f[6] += mt * (terms[0] + (-uz3 + uz3uz3d2));
-        mt = rhoW2;
+        mt = eqs * rhoW2;
         f[7] += mt * (terms[1] + (uy3 + ux3uy3 + uy3uy3d2));
         f[8] += mt * (terms[2] + (-uy3 + ux3uy3 + uy3uy3d2));
         f[9] += mt * (terms[1] + (uz3 + ux3uz3 + uz3uz3d2));
@@ -649,7 +665,7 @@ struct Lattice {
         f[17] += mt * (terms[3] + (-uz3 - uy3uz3 + uz3uz3d2));
         f[18] += mt * (terms[4] + (uz3 - uy3uz3 + uz3uz3d2));
         if (q27) {
-            mt = rhoW3;
+            mt = eqs * rhoW3;
             f[19] += mt * (terms[5] + (uz3 + ux3uz3 + uy3uz3 + uz3uz3d2));
             f[20] += mt * (terms[6] + (-uz3 + ux3uz3 + uy3uz3 + uz3uz3d2));
             f[21] += mt * (terms[5] + (-uz3 - ux3uz3 - uy3uz3 + uz3uz3d2));

@@ -35,6 +35,7 @@ class OrDesc(C.Structure):
         ("duct_outer_radius", C.c_double), ("duct_inner_radius", C.c_double),
         ("outer_rotation", C.c_double), ("outer_velocity", C.c_double),
         ("inner_rotation", C.c_double), ("inner_velocity", C.c_double),
+        ("collision", C.c_int), ("les_const", C.c_double),
     ]
 
 

@@ -104,6 +104,26 @@ CASES_FP64 = [
      120, 0.0),
     ("annular_duct_ibb_q27", dict(NX=22, NY=22, NZ=3, tau=0.8, FZ=1e-5, vel_set="D3Q27",
                                   builder="annularDuctInterpBounceBack", inner_rotation=0.002), 80, 0.0),
+    # BGK collision (collisionSchemes/recursiveCollisionForceNoReg.txt) with each rheology. In Bingham
+    # plugs (omega = 0) BGK leaves the non-hydrodynamic modes undamped and this noisy channel turns
+    # chaotic after ~150 steps (GPU and oracle agree to 1e-12 up to step 100, then round-off grows
+    # exponentially on both), so the case runs 100 steps.
+    ("plates_bingham_bgk", dict(NX=4, NY=32, NZ=6, tau=0.9, FX=1e-5, builder="parallelPlatesBounceBack_Y",
+                                non_newtonian_model="BINGHAM", yield_stress=4e-5, collision="bgk"), 100, 1e-3),
+    ("closed_box_q27_bgk", dict(NX=8, NY=7, NZ=6, tau=0.7, vel_set="D3Q27", builder="closedBoxBounceBack",
+                                collision="bgk"), 60, 1e-2),
+    ("square_duct_zouhe_bgk", dict(NX=12, NY=10, NZ=16, tau=0.8, builder="squareDuctZouHe", collision="bgk",
+                                   uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0]), 150, 1e-4),
+    ("taylor_green_q27_power_law_bgk", dict(NX=8, NY=8, NZ=8, tau=0.7, vel_set="D3Q27", builder="taylorGreenVortex",
+                                            non_newtonian_model="POWERLAW", n_index=0.6, k_consistency=0.05,
+                                            tau_clamp=[0.51, 2.0], collision="bgk"), 80, 5e-3),
+    ("circular_duct_ibb_bgk", dict(NX=18, NY=18, NZ=4, tau=0.8, FZ=1e-5, builder="circularDuctInterpBounceBack",
+                                   collision="bgk"), 120, 0.0),
+    # LES (implicit Smagorinsky, lbm.cu:226-242), regularized and BGK
+    ("lid_driven_les", dict(NX=12, NY=12, NZ=6, tau=0.52, builder="lidDrivenBounceBack", u_max=0.1,
+                            les_const=25.0), 150, 0.0),
+    ("closed_box_q27_les_bgk", dict(NX=8, NY=7, NZ=6, tau=0.51, vel_set="D3Q27", builder="closedBoxBounceBack",
+                                    les_const=10.0, collision="bgk"), 60, 2e-2),
 ]
 
 
@@ -126,7 +146,8 @@ def test_fp64_matches_oracle(name, kw, steps, noise):
 
 @pytest.mark.parametrize("name,kw,steps,noise", [c for c in CASES_FP64 if c[0] in
                                                   ("plates_bingham_c1", "square_duct_zouhe_c2", "closed_box_q27",
-                                                   "plates_power_law_q27", "circular_duct_ibb")],
+                                                   "plates_power_law_q27", "circular_duct_ibb",
+                                                   "square_duct_zouhe_bgk", "lid_driven_les")],
                          ids=lambda v: v if isinstance(v, str) else "")
 def test_fp32_within_1e4_of_fp64_oracle(name, kw, steps, noise):
     announce("fp32 " + name)

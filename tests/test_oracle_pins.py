@@ -158,6 +158,71 @@ def test_collision_power_law_omega(clamp):
     assert np.isclose(m[4], min(max(om0, 0.5), 1 / 0.51) if clamp else om0)
 
 
+def bgk_collision(q, f, F, omega):
+    """Independent statement of BGK with Guo forcing:
+    f* = f - omega (f - feq) + (1 - omega/2) w_i [3 (c - u).F + 9 (c.u)(c.F)]."""
+    c, w = cvec(q).astype(float), weights(q)
+    rho = f.sum()
+    u = (f @ c + 0.5 * np.asarray(F)) / rho
+    cf, cu = c @ F, c @ u
+    guo = w * (3 * (cf - np.dot(u, F)) + 9 * cu * cf)
+    return f - omega * (f - feq(q, rho, u)) + (1 - omega / 2) * guo
+
+
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("model", [None, "BINGHAM", "POWERLAW"])
+def test_collision_bgk_matches_closed_form(q, model):
+    """BGK keeps the non-equilibrium part of every population (it is not
+    projected on Pi^neq): f* - f = -omega (f - feq) + force term, with the
+    rheology's apparent omega; mass kept, momentum gains F."""
+    kw = dict(NX=1, NY=1, NZ=1, tau=0.8, vel_set="D3Q19" if q == 19 else "D3Q27", builder="taylorGreenVortex",
+              non_newtonian_model=model, collision="bgk", yield_stress=2e-5, n_index=0.7, k_consistency=0.02)
+    o = oracle_for(base_cfg(**kw))
+    reg = oracle_for(base_cfg(**{**kw, "collision": "regularized"}))
+    rng = np.random.default_rng(13)
+    c = cvec(q)
+    for k in range(10):
+        f = feq(q, 1.0 + 0.01 * rng.standard_normal(), 0.02 * rng.standard_normal(3))
+        f = f + 1e-4 * rng.standard_normal(q) * weights(q)
+        F = 1e-4 * rng.standard_normal(3)
+        om_old = 1.1 if model == "POWERLAW" else 0.0
+        out, m = o.collide_node(f, *F, omega_old=om_old)
+        _, mr = reg.collide_node(f, *F, omega_old=om_old)
+        assert m[4] == mr[4]                      # same apparent omega in both schemes
+        exp = bgk_collision(q, f, F, m[4])
+        assert np.allclose(out, exp, rtol=0, atol=1e-15)
+        assert np.isclose(out.sum(), f.sum(), rtol=0, atol=1e-14)
+        assert np.allclose(out @ c, f @ c + F, atol=1e-15)
+    # the regularized scheme differs exactly by the ghost (non-hydrodynamic) part
+    fr, _ = reg.collide_node(f, *F, omega_old=om_old)
+    assert np.abs(fr - out).max() > 1e-9
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_collision_les_omega(q):
+    """Implicit Smagorinsky: tau_t = (sqrt(tau^2 + C |S|) - tau) / 2, omega =
+    1/(tau + tau_t), with |S| the same stress magnitude as the non-Newtonian
+    models; C = 0 is the Newtonian omega."""
+    tau, C = 0.55, 25.0
+    kw = dict(NX=1, NY=1, NZ=1, tau=tau, vel_set="D3Q19" if q == 19 else "D3Q27", builder="taylorGreenVortex")
+    o = oracle_for(base_cfg(les_const=C, **kw))
+    o0 = oracle_for(base_cfg(**kw))
+    rng = np.random.default_rng(21)
+    for k in range(20):
+        f = feq(q, 1.0, 0.03 * rng.standard_normal(3)) + 10 ** rng.uniform(-5, -2) * rng.standard_normal(q) * weights(q)
+        F = 1e-5 * rng.standard_normal(3)
+        out, m = o.collide_node(f, *F)
+        _, rho, u, pi = hermite_collision(q, f, F, 1.0)
+        smag = stress_mag(pi, u, F)
+        tau_t = 0.5 * np.sqrt(tau * tau + C * smag) - 0.5 * tau
+        assert np.isclose(m[4], 1 / (tau + tau_t), rtol=1e-12)
+        assert m[4] < 1 / tau                      # eddy viscosity only adds
+        exp, *_ = hermite_collision(q, f, F, m[4])
+        assert np.allclose(out, exp, atol=1e-15)
+        _, m0 = o0.collide_node(f, *F)
+        assert m0[4] == 1 / tau
+
+
 # ---------------------------------------------------------------------------
 # boundary schemes on random data (src/CUDA/boundaryConditionsSchemes/*.cu)
 # ---------------------------------------------------------------------------

@@ -195,6 +195,12 @@ int validate(const vlbm_desc* d) {
     if (d->rheology == VLBM_POWERLAW && !(d->k_consistency > 0)) return fail(VLBM_ERR_INVALID, "K must be > 0");
     if (d->omega_clamp && !(d->tau_min > 0.5 && d->tau_max >= d->tau_min))
         return fail(VLBM_ERR_INVALID, "tau clamp needs 0.5 < tau_min <= tau_max");
+    if (d->collision != VLBM_COLLISION_REGULARIZED && d->collision != VLBM_COLLISION_BGK)
+        return fail(VLBM_ERR_INVALID, "collision must be VLBM_COLLISION_REGULARIZED or VLBM_COLLISION_BGK");
+    if (!(d->les_const >= 0)) return fail(VLBM_ERR_INVALID, "LES constant must be >= 0");
+    if (d->les_const > 0 && d->rheology != VLBM_NEWTONIAN)
+        return fail(VLBM_ERR_UNSUPPORTED, "LES applies to Newtonian fluids only");
+    if (d->les_const > 0 && !(d->tau > 0.5)) return fail(VLBM_ERR_INVALID, "LES needs tau > 0.5");
     if (d->q == 27 && d->builder != VLBM_BUILDER_TAYLOR_GREEN) {
         // Zou-He and velocity bounce-back exist for D3Q19 only in the reference
         // (boundaryConditionsHandler.cu:274,318,361); refuse instead of silently skipping.

@@ -93,6 +93,9 @@ inline void fill_params_desc(const vlbm_desc& d, const uint32_t region_word[27],
     p.clamp = d.omega_clamp;
     if (d.omega_clamp) { p.omega_lo = (T)(1.0 / d.tau_max); p.omega_hi = (T)(1.0 / d.tau_min); }
     p.zouhe_textbook = d.zouhe_textbook;
+    p.bgk = d.collision == VLBM_COLLISION_BGK ? 1 : 0;
+    p.tau = (T)d.tau;
+    p.les_c = (T)d.les_const;
     p.duct_R = (T)d.duct_outer_radius; p.duct_r = (T)d.duct_inner_radius;
     p.w_out = (T)d.outer_rotation; p.u_out = (T)d.outer_velocity;
     p.w_in = (T)d.inner_rotation; p.u_in = (T)d.inner_velocity;

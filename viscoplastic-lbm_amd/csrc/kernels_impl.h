@@ -439,14 +439,19 @@ __device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<
     m.pyz = syz - rho * uy * uz;
     m.dr = dr; m.rho = rho; m.ux = ux; m.uy = uy; m.uz = uz;
     T om = p.omega;
-    if (RHEO != 0) {
+    if (RHEO != 0 || p.les_c != (T)0) {
         // |Pi^neq + sym(u F)| (src/CUDA/lbm.cu:192-208)
         const T axx = m.pxx + ux * F.x, ayy = m.pyy + uy * F.y, azz = m.pzz + uz * F.z;
         const T axy = m.pxy + (ux * F.y + uy * F.x) * (T)0.5;
         const T axz = m.pxz + (ux * F.z + uz * F.x) * (T)0.5;
         const T ayz = m.pyz + (uy * F.z + uz * F.y) * (T)0.5;
         const T smag = sqrt((T)0.5 * (axx * axx + ayy * ayy + azz * azz + 2 * (axy * axy + axz * axz + ayz * ayz)));
-        if (RHEO == 1) {
+        if (RHEO == 0) {
+            // implicit Smagorinsky closure (LES_MODEL, src/CUDA/lbm.cu:226-242):
+            // tau_t = (sqrt(tau^2 + C |S|) - tau) / 2, omega = 1 / (tau + tau_t)
+            const T tau_t = (T)0.5 * sqrt(p.tau * p.tau + p.les_c * smag) - (T)0.5 * p.tau;
+            om = (T)1 / (p.tau + tau_t);
+        } else if (RHEO == 1) {
             // Bingham: omega = omega_p max(0, 1 - tau_y/|S|) (src/CUDA/NNF/nnf.h:83);
             // a zero yield stress is the Newtonian plastic fluid.
             if (p.s_y == (T)0) {
@@ -462,7 +467,7 @@ __device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<
             eta = (gdot <= p.gamma_0) ? p.k_cons : p.k_cons * pow(gdot, p.n_minus_1);
             om = (T)1 / ((T)0.5 + (T)3 * eta);
         }
-        if (p.clamp) {
+        if (RHEO != 0 && p.clamp) {
             // tau in [tau_min, tau_max]  <=>  omega in [omega_lo, omega_hi]
             if (!(om <= p.omega_hi)) om = p.omega_hi;
             else if (om < p.omega_lo) om = p.omega_lo;
@@ -506,6 +511,32 @@ __device__ __forceinline__ void collide(T* g, const Moments<Q, RHEO, T>& m, cons
     }
 }
 
+// BGK collision with Guo forcing in deviation form
+// (src/CUDA/collisionSchemes/recursiveCollisionForceNoReg.txt, with the
+// apparent omega of the rheology):
+//   g*_i = (1-omega) g_i + omega w_i [ (rho-1) + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u.u) ]
+//        + (1-omega/2) w_i (3 c.F + 9 (c.u)(c.F) - 3 u.F)
+template <int Q, int RHEO, typename T>
+__device__ __forceinline__ void collide_bgk(T* g, const Moments<Q, RHEO, T>& m, const Force<T>& F) {
+    const T om = m.omega;
+    const T keep = (T)1 - om;
+    const T b = (T)1 - (T)0.5 * om;
+    const T uf = m.ux * F.x + m.uy * F.y + m.uz * F.z;
+    const T usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
+    const T e0 = m.dr - (T)1.5 * m.rho * usq;
+    const T f0 = -(T)3 * uf;
+    const T r3 = (T)3 * m.rho, r45 = (T)4.5 * m.rho;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+        const T cu = (T)cx * m.ux + (T)cy * m.uy + (T)cz * m.uz;
+        const T cf = (T)cx * F.x + (T)cy * F.y + (T)cz * F.z;
+        const T eq = e0 + cu * (r3 + r45 * cu);
+        const T fo = f0 + (T)3 * cf + (T)9 * cu * cf;
+        g[i] = keep * g[i] + (T)weight<Q>(i) * (om * eq + b * fo);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
@@ -537,7 +568,8 @@ __device__ __forceinline__ void step_node(const Params<typename ComputeOf<S>::ty
         p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
         if (RHEO != 0) p.m_omega[idx] = m.omega;
     }
-    collide<Q, RHEO, T>(g, m, F);
+    if (p.bgk) collide_bgk<Q, RHEO, T>(g, m, F);
+    else collide<Q, RHEO, T>(g, m, F);
     S* dst = (S*)p.dst;
 #pragma unroll
     for (int i = 0; i < Q; i++) st_nt<S, T>(dst + (int64_t)i * p.stride + idx, g[i]);

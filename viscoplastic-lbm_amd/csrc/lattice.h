@@ -121,6 +121,8 @@ struct Params {
     T* m_rho; T* m_ux; T* m_uy; T* m_uz; T* m_omega;
     int save;
     int zouhe_textbook;
+    int bgk;                 // BGK collision instead of the regularized one
+    T tau, les_c;            // LES: tau_t = (sqrt(tau^2 + les_c |S|) - tau) / 2 when les_c != 0
     int aux_mode;               // aux kernel: kAuxMoments / kAuxMaterialize
     int direct;                 // aux kernel: state is post-stream (no gather, no BC)
     T duct_R, duct_r, w_out, u_out, w_in, u_in;

@@ -37,6 +37,8 @@ class VlbmDesc(C.Structure):
         ("outer_rotation", C.c_double), ("outer_velocity", C.c_double),
         ("inner_rotation", C.c_double), ("inner_velocity", C.c_double),
         ("zouhe_textbook", C.c_int),
+        ("collision", C.c_int),
+        ("les_const", C.c_double),
     ]
 
 

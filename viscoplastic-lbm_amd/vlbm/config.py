@@ -21,6 +21,8 @@ file); here they are optional extra keys with the reference defaults:
     duct_outer_radius, duct_inner_radius, outer_rotation, outer_velocity,
     inner_rotation, inner_velocity        # duct builders
     zouhe_textbook: false                 # textbook Zou-He instead of the reference lines
+    collision: regularized                # or bgk (collisionSchemes/recursiveCollisionForceNoReg.txt)
+    les_const: 0.0                        # LES_MODEL Implicit_const, Newtonian only (lbm.cu:226-242)
     checkpoint_save: 0                    # CHECKPOINT_SAVE interval (var.h:69-71)
     load_checkpoint: null                 # folder to resume from: LOAD_CHECKPOINT with
                                           # SIMULATION_FOLDER_LOAD_CHECKPOINT (var.h:77-84)
@@ -54,6 +56,7 @@ BUILDERS = {
 
 PRECISIONS = {"double": _capi.FP64, "single": _capi.FP32, "half": _capi.FP16}
 MODELS = {None: _capi.NEWTONIAN, "BINGHAM": _capi.BINGHAM, "POWERLAW": _capi.POWERLAW}
+COLLISIONS = {"regularized": 0, "bgk": 1}
 
 
 @dataclass
@@ -102,6 +105,8 @@ class SimConfig:
     inner_rotation: float = 0.0
     inner_velocity: float = 0.0
     zouhe_textbook: bool = False
+    collision: str = "regularized"
+    les_const: float = 0.0
     checkpoint_save: int = 0
     load_checkpoint: Optional[str] = None
     # immersed boundary (use_ibm): vlbm_ibm_desc fields (ibmVar.h /
@@ -167,6 +172,8 @@ class SimConfig:
         d.outer_rotation, d.outer_velocity = self.outer_rotation, self.outer_velocity
         d.inner_rotation, d.inner_velocity = self.inner_rotation, self.inner_velocity
         d.zouhe_textbook = 1 if self.zouhe_textbook else 0
+        d.collision = COLLISIONS[self.collision]
+        d.les_const = self.les_const
         return d
 
     def replace(self, **kw) -> "SimConfig":
@@ -190,7 +197,7 @@ def _validate(c: SimConfig) -> None:
         need(isinstance(v, int) and not isinstance(v, bool), f"{name} must be an integer")
         setattr(c, name, v)
     floats = ("tau", "FX", "FY", "FZ", "resid_max", "rho_0", "u_max", "yield_stress", "n_index", "gamma_0",
-              "outer_rotation", "outer_velocity", "inner_rotation", "inner_velocity")
+              "outer_rotation", "outer_velocity", "inner_rotation", "inner_velocity", "les_const")
     opt_floats = ("eta_p", "k_consistency", "duct_outer_radius", "duct_inner_radius")
     for name in floats + opt_floats:
         v = getattr(c, name)
@@ -206,6 +213,9 @@ def _validate(c: SimConfig) -> None:
     for name in ("use_ibm", "data_stop", "data_save", "pop_save", "zouhe_textbook"):
         need(isinstance(getattr(c, name), bool), f"{name} must be a boolean")
     need(c.checkpoint_save >= 0, "checkpoint_save must be >= 0")
+    need(c.collision in COLLISIONS, "collision must be 'regularized' or 'bgk'")
+    need(c.les_const >= 0, "les_const must be >= 0")
+    need(c.les_const == 0 or c.non_newtonian_model is None, "LES applies to Newtonian fluids only")
     need(c.load_checkpoint is None or isinstance(c.load_checkpoint, str), "load_checkpoint must be a folder name")
     need(c.precision in PRECISIONS, "precision must be 'single' or 'double' (or 'half')")
     need(c.vel_set in ("D3Q19", "D3Q27"), "vel_set must be 'D3Q19' or 'D3Q27'")

@@ -300,14 +300,21 @@ def sweep(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every=1000
     return res
 
 
-def sequential(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every=1000, log=None) -> SweepResult:
+def sequential(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every=1000, log=None,
+               realtime=True) -> SweepResult:
     """The same jobs one at a time through the plain single-lattice path,
     with the same stop test and check cadence (first step direct, then
-    checks after 1 + k * check_every steps)."""
+    checks after 1 + k * check_every steps), in arrival order: a job starts
+    when it has arrived and the previous one is done; latency is measured
+    from its arrival."""
     from .lattice import Lattice
     res = SweepResult(jobs=len(jobs), seconds=0.0, node_updates=0.0)
     t_start = time.perf_counter()
     for k, spec in enumerate(jobs):
+        if realtime:
+            wait = spec.arrival - (time.perf_counter() - t_start)
+            if wait > 0:
+                time.sleep(wait)
         lat = Lattice(spec.desc())
         lat.init()
         steps = 0
@@ -331,7 +338,7 @@ def sequential(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every
         m = lat.macroscopics()
         lat.close()
         t_done = time.perf_counter() - t_start
-        res.latencies.append(t_done - 0.0)
+        res.latencies.append(t_done - (spec.arrival if realtime else 0.0))
         res.errors.append(profile_error(spec, m["ux"]))
         res.node_updates += float(spec.nx * spec.ny * spec.nz) * steps
         res.states[state] = res.states.get(state, 0) + 1
