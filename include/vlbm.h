@@ -42,6 +42,16 @@ extern "C" {
 #define VLBM_COLLISION_REGULARIZED 0
 #define VLBM_COLLISION_BGK 1
 
+/* State layout in HBM. POPULATIONS: Q planes of post-collision deviations.
+ * MOMENTS: 10 planes of coefficients (A, B, G) with g*_i = w_i (A + B.c_i +
+ * c_i^T G c_i), which hold the regularized post-collision state exactly (float
+ * and double storage, regularized collision only): 10 instead of Q values
+ * read and written per node. AUTO: MOMENTS for D3Q27 in float storage with
+ * NX >= 64, else POPULATIONS. */
+#define VLBM_LAYOUT_AUTO 0
+#define VLBM_LAYOUT_POPULATIONS 1
+#define VLBM_LAYOUT_MOMENTS 2
+
 /* boundary-condition builders (src/CUDA/boundaryConditionsBuilders/) */
 #define VLBM_BUILDER_TAYLOR_GREEN 0          /* taylorGreenVortex.cu: fully periodic */
 #define VLBM_BUILDER_PLATES_BB_Y 1           /* parallelPlatesBounceBack_Y.cu */
@@ -83,6 +93,7 @@ typedef struct vlbm_desc {
     int zouhe_textbook;      /* 0: reproduce the reference's Zou-He lines; 1: textbook Zou-He */
     int collision;           /* VLBM_COLLISION_REGULARIZED or VLBM_COLLISION_BGK */
     double les_const;        /* LES_MODEL's Implicit_const (lbm.cu:226-242); 0 = no LES, Newtonian only */
+    int layout;              /* VLBM_LAYOUT_* (not in the reference: its state is always populations) */
 } vlbm_desc;
 
 typedef struct vlbm_ctx* vlbm_h;
@@ -310,6 +321,8 @@ typedef struct vlbm_batch_desc {
     int grid;                /* blocks of the persistent grid; 0 = autotune */
     int use_graph;           /* 1: replay captured sequences (hipGraph); 0: plain launches */
     const char* tune_cache;  /* file the autotuner reads and appends (NULL: none) */
+    size_t max_resident_nodes;   /* admit a pending job only while the resident lattices hold at most
+                                    this many nodes (a lone job is always admitted); 0 = no limit */
 } vlbm_batch_desc;
 
 #define VLBM_JOB_PENDING 0

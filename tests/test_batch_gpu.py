@@ -142,3 +142,32 @@ def test_converged_channel_matches_analytic_profile():
         err = profile_error(s, res["ux"])
         print("profile error", k, err, inf["steps"])
         assert err < 0.03
+
+
+def test_node_budget_limits_residency_and_keeps_results():
+    """max_resident_nodes admits a pending job only while the resident
+    lattices stay within the node budget; results are the same as with every
+    job resident at once (each job's arithmetic is independent of the mix)."""
+    from vlbm.serve import Batch
+    jobs = [spec(24, 20, 12, 1.0, 0.8, 5e-6), spec(24, 20, 12, 2.0, 0.9, 4e-6), spec(30, 18, 10, 0.0, 0.7, 6e-6),
+            spec(16, 22, 14, 3.0, 1.1, 8e-6)]
+    n = [s.nx * s.ny * s.nz for s in jobs]
+    print("case: node budget", flush=True)
+    free, _ = run_batch(jobs, max_steps=80)
+    b = Batch(check_every=20, steps_per_launch=10, block=256, grid=64, hbm_budget=1 << 30,
+              max_resident_nodes=n[0] + n[1])
+    ids = [b.submit(j.desc(), 0.0, 80) for j in jobs]
+    running, pending, _ = b.advance()
+    assert running == 2 and pending == 2
+    calls = 0
+    while any(b.info(i)["state"] in ("pending", "running") for i in ids):
+        r, p, _ = b.advance()
+        assert r <= 2
+        calls += 1
+        assert calls < 500
+    for k, i in enumerate(ids):
+        res = b.result(i, (jobs[k].nz, jobs[k].ny, jobs[k].nx))
+        assert b.info(i)["steps"] == free[k][0]["steps"]
+        for f in ("rho", "ux", "uy", "uz"):
+            assert np.array_equal(res[f], free[k][1][f]), (k, f)
+    b.close()

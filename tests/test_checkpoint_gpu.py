@@ -122,12 +122,13 @@ def test_checkpoint_files_match_oracle_and_oracle_continues(name, kw, tmp_path):
     lat.close()
 
 
+@pytest.mark.parametrize("layout", ["populations", "moments"])
 @pytest.mark.parametrize("precision,tol", [("double", 1e-12), ("single", 1e-4)])
 @pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
-def test_loaded_lattice_continues_the_run(name, kw, precision, tol, tmp_path):
+def test_loaded_lattice_continues_the_run(name, kw, precision, tol, layout, tmp_path):
     import vlbm
-    announce(f"checkpoint resume: {name} {precision}")
-    cfg = base_cfg(precision=precision, **kw)
+    announce(f"checkpoint resume: {name} {precision} [{layout}]")
+    cfg = base_cfg(precision=precision, layout=layout, **kw)
     q = 19 if cfg.vel_set == "D3Q19" else 27
     a = vlbm.Lattice(cfg.to_desc())
     a.init(*noisy(cfg))
@@ -178,11 +179,11 @@ def test_load_rejects_a_checkpoint_of_another_lattice(tmp_path):
 # ---------------------------------------------------------------------------
 # immersed boundary
 # ---------------------------------------------------------------------------
-def ibm_case(precision="double"):
+def ibm_case(precision="double", layout="auto"):
     import vlbm
     from vlbm import _capi
     cfg = base_cfg(NX=32, NY=24, NZ=32, builder="parallelPlatesBounceBack_Y", tau=0.6, precision=precision,
-                   FX=1e-5)
+                   FX=1e-5, layout=layout)
     idesc = _capi.default_ibm_desc(max_iterations=2, gy=-2e-5, young=5.0)
     spheres = [([10.0, 9.0, 12.0], 4.0, (0.001, 0, 0), (0, 0, 0)),
                ([17.5, 9.3, 12.4], 4.0, (0, 0, 0), (0, 0.002, 0)),
@@ -274,10 +275,11 @@ def test_ibm_checkpoint_files_match_oracle(tmp_path):
     lat.close()
 
 
+@pytest.mark.parametrize("layout", ["populations", "moments"])
 @pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-4)])
-def test_ibm_loaded_lattice_continues_the_run(precision, tol, tmp_path):
-    announce(f"IBM checkpoint resume {precision}")
-    cfg, make, make_oracle = ibm_case(precision)
+def test_ibm_loaded_lattice_continues_the_run(precision, tol, layout, tmp_path):
+    announce(f"IBM checkpoint resume {precision} [{layout}]")
+    cfg, make, make_oracle = ibm_case(precision, layout)
     a = make()
     a.step(8)
     a.save_checkpoint(str(tmp_path), "000")

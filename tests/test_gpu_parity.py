@@ -127,10 +127,16 @@ CASES_FP64 = [
 ]
 
 
+LAYOUTS = ["populations", "moments"]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("name,kw,steps,noise", CASES_FP64, ids=[c[0] for c in CASES_FP64])
-def test_fp64_matches_oracle(name, kw, steps, noise):
-    announce(name)
-    cfg = base_cfg(**kw)
+def test_fp64_matches_oracle(name, kw, steps, noise, layout):
+    if layout == "moments" and kw.get("collision") == "bgk":
+        pytest.skip("the moment layout holds the regularized state only")
+    announce(f"{name} [{layout}]")
+    cfg = base_cfg(layout=layout, **kw)
     shape = (cfg.NZ, cfg.NY, cfg.NX)
     init = None
     if noise:
@@ -149,9 +155,12 @@ def test_fp64_matches_oracle(name, kw, steps, noise):
                                                    "plates_power_law_q27", "circular_duct_ibb",
                                                    "square_duct_zouhe_bgk", "lid_driven_les")],
                          ids=lambda v: v if isinstance(v, str) else "")
-def test_fp32_within_1e4_of_fp64_oracle(name, kw, steps, noise):
-    announce("fp32 " + name)
-    cfg = base_cfg(**kw)
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_fp32_within_1e4_of_fp64_oracle(name, kw, steps, noise, layout):
+    if layout == "moments" and kw.get("collision") == "bgk":
+        pytest.skip("the moment layout holds the regularized state only")
+    announce(f"fp32 {name} [{layout}]")
+    cfg = base_cfg(layout=layout, **kw)
     shape = (cfg.NZ, cfg.NY, cfg.NX)
     init = None
     if noise:
@@ -176,10 +185,11 @@ def test_fp16_deviation_storage_plates():
     lat.close()
 
 
-def test_state_roundtrip_and_on_demand_moments():
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_state_roundtrip_and_on_demand_moments(layout):
     """load_pops/get_pops round trip; moments on demand equal the next step's saved ones."""
-    announce("roundtrip")
-    cfg = base_cfg(NX=6, NY=7, NZ=5, tau=0.8, FX=1e-5, builder="squareDuctBounceBack")
+    announce(f"roundtrip [{layout}]")
+    cfg = base_cfg(NX=6, NY=7, NZ=5, tau=0.8, FX=1e-5, builder="squareDuctBounceBack", layout=layout)
     shape = (5, 7, 6)
     lat, orc = run_both(cfg, 25, (np.ones(shape),) + seeded_velocity(shape, (0, 0, 0), 1e-2, 3))
     pops = orc.get_pops()
@@ -261,6 +271,32 @@ def test_c2_fp32_vs_fp64_full_size():
     u32 = np.stack([out["single"][k] for k in ("ux", "uy", "uz")])
     u64 = np.stack([out["double"][k] for k in ("ux", "uy", "uz")])
     assert rel_l2(u32, u64) <= 1e-4
+
+
+def test_c2_layouts_agree_full_size():
+    """Both state layouts hold the regularized post-collision state exactly:
+    at 256^3 after 300 steps the coefficient layout equals the population
+    layout to FP64 round-off (1e-12), and FP32 coefficients stay within 1e-4
+    of FP64."""
+    announce("c2 256^3 layouts, 300 steps")
+    shape = (256, 256, 256)
+    init = (np.ones(shape),) + seeded_velocity(shape, (0, 0, 0.01), 1e-4, seed=12345)
+    out = {}
+    for prec, layout in (("double", "populations"), ("double", "moments"), ("single", "moments")):
+        cfg = c2_config(prec).replace(layout=layout)
+        lat = vlbm.Lattice(cfg.to_desc())
+        lat.init(*init)
+        lat.step(300, save_last=True)
+        out[(prec, layout)] = lat.macroscopics()
+        lat.close()
+    ref = out[("double", "populations")]
+    for key, tol in ((("double", "moments"), 1e-12), (("single", "moments"), 1e-4)):
+        m = out[key]
+        assert rel_l2(m["rho"], ref["rho"]) <= tol, key
+        u = np.stack([m[k] for k in ("ux", "uy", "uz")])
+        u0 = np.stack([ref[k] for k in ("ux", "uy", "uz")])
+        print(key, "u rel-L2", rel_l2(u, u0))
+        assert rel_l2(u, u0) <= tol, key
 
 
 # ---------------------------------------------------------------------------

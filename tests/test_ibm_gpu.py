@@ -73,12 +73,13 @@ def compare_fluid(lat, o, tol):
     assert e_rho <= tol and e_u <= tol and e_f <= tol * 10
 
 
-def test_ibm_fp64_contacts_walls_and_iterations():
+@pytest.mark.parametrize("layout", ["populations", "moments"])
+def test_ibm_fp64_contacts_walls_and_iterations(layout):
     """Two overlapping spheres and one pressed into the y = 0 wall, gravity,
     two IBM iterations per step, periodic x / z: 12 coupled steps."""
     ny = 24
     lat, o = make_pair(32, ny, 32, "parallelPlatesBounceBack_Y",
-                       dict(max_iterations=2, gy=-2e-5, young=5.0), FX=1e-5)
+                       dict(max_iterations=2, gy=-2e-5, young=5.0), FX=1e-5, layout=layout)
     add_both(lat, o, [10.0, 9.0, 12.0], 4.0, vel=(0.001, 0, 0))
     add_both(lat, o, [17.5, 9.3, 12.4], 4.0, w=(0, 0.002, 0))
     add_both(lat, o, [24.2, 3.8, 21.7], 4.0, vel=(0, -0.002, 0.001))

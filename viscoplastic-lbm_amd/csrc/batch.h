@@ -27,13 +27,17 @@ struct BatchJob {
     int pad;
 };
 
-// Launchers (kernels_batch.hip). prec 8 / 4 / 2; block 128 or 256.
-int launch_batch_step(int q, int prec, int block, uint32_t grid, const void* jobs, const uint32_t* prefix,
-                      const uint32_t* n_jobs, int k, void* stream);
+// Launchers (kernels_batch.hip). prec 8 / 4 / 2; block 128 or 256. Work
+// unit w (BLOCK nodes) belongs to job unit_job[w] and starts at node
+// (w - prefix[job]) * BLOCK of it. rheo: the rheology every resident job
+// shares, or 3 for a mix.
+int launch_batch_step(int q, int prec, int block, int rheo, uint32_t grid, const void* jobs, const uint32_t* prefix,
+                      const uint16_t* unit_job, const uint32_t* n_jobs, int k, void* stream);
 // per-job sums {sum |u - u_old|, sum |u|, sum rho} of the jobs flagged due;
 // u_old <- u for them
 int launch_batch_check(int q, int prec, int block, uint32_t grid, const void* jobs, const uint32_t* prefix,
-                       const uint32_t* n_jobs, uint32_t n_jobs_host, double* partials, double* sums, void* stream);
+                       const uint16_t* unit_job, const uint32_t* n_jobs, uint32_t n_jobs_host, double* partials,
+                       double* sums, void* stream);
 // rest state g_i = w_i (rho0 - 1) in deviation storage
 int launch_batch_fill_rest(int q, int prec, void* lat, uint32_t n, double rho0, void* stream);
 

@@ -66,6 +66,10 @@ struct vlbm_batch {
     std::vector<int> running;                  // admission order = table order
     void* d_jobs = nullptr;                    // BatchJob<T>[max_jobs]
     uint32_t* d_prefix = nullptr;              // [max_jobs + 1]
+    uint16_t* d_unit_job = nullptr;            // job slot of every work unit
+    size_t unit_cap = 0;                       // capacity of d_unit_job (bounded by the arena)
+    bool units_dirty = true;                   // running set changed since the last upload
+    size_t resident_nodes = 0;
     uint32_t* d_njobs = nullptr;
     double* d_partials = nullptr;
     size_t partials_cap = 0;                   // work units
@@ -76,7 +80,7 @@ struct vlbm_batch {
     bool tuned = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_K = 0, graph_block = 0, graph_grid = 0;
+    int graph_K = 0, graph_block = 0, graph_grid = 0, graph_rheo = -1;
     std::chrono::steady_clock::time_point t0;
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
@@ -172,6 +176,15 @@ int upload_table(vlbm_batch* b, const std::vector<int>& due) {
         b->prefix[k + 1] = b->prefix[k] + (n + b->block - 1) / b->block;
     }
     const uint32_t units = b->prefix[nj];
+    if (units > b->unit_cap) return fail(VLBM_ERR_INVALID, "work units exceed the table capacity");
+    if (b->units_dirty && units) {
+        std::vector<uint16_t> uj(units);
+        for (size_t k = 0; k < nj; k++)
+            for (uint32_t w = b->prefix[k]; w < b->prefix[k + 1]; w++) uj[w] = (uint16_t)k;
+        HIP_TRY(hipMemcpyAsync(b->d_unit_job, uj.data(), (size_t)units * 2, hipMemcpyHostToDevice, b->stream));
+        if (int rc = sync(b)) return rc;
+    }
+    b->units_dirty = false;
     if (units > b->partials_cap) {
         if (b->d_partials) (void)hipFree(b->d_partials);
         b->d_partials = nullptr;
@@ -219,7 +232,9 @@ int admit(vlbm_batch* b, int id) {
     j.state = VLBM_JOB_RUNNING;
     j.t_start = b->now();
     b->running.push_back(id);
+    b->resident_nodes += n;
     b->table_dirty = true;
+    b->units_dirty = true;
     return VLBM_OK;
 }
 
@@ -265,7 +280,9 @@ int retire(vlbm_batch* b, int id, int state) {
     j.t_end = b->now();
     arena_free(b, j.off, j.bytes);
     b->running.erase(std::find(b->running.begin(), b->running.end(), id));
+    b->resident_nodes -= n;
     b->table_dirty = true;
+    b->units_dirty = true;
     return VLBM_OK;
 }
 
@@ -274,6 +291,13 @@ int admit_pending(vlbm_batch* b) {
     for (auto it = b->pending.begin(); it != b->pending.end();) {
         if ((int)b->running.size() >= b->d.max_jobs) break;
         Job& j = b->jobs[*it];
+        const size_t nodes = (size_t)j.d.nx * j.d.ny * j.d.nz;
+        // node budget: enough resident work to fill the GPU, not more, so that
+        // jobs finish in arrival order instead of all together
+        if (b->d.max_resident_nodes && !b->running.empty() && b->resident_nodes + nodes > b->d.max_resident_nodes) {
+            ++it;
+            continue;
+        }
         size_t off;
         if (!arena_alloc(b, j.bytes, &off)) { ++it; continue; }
         j.off = off;
@@ -285,14 +309,27 @@ int admit_pending(vlbm_batch* b) {
     return VLBM_OK;
 }
 
+// rheology shared by every running job (the step kernel is specialised to
+// it), or 3 for a mix
+int common_rheo(const vlbm_batch* b) {
+    int r = -1;
+    for (int id : b->running) {
+        const int jr = b->jobs[id].d.rheology;
+        if (r < 0) r = jr;
+        else if (r != jr) return 3;
+    }
+    return r < 0 ? 0 : r;
+}
+
 int build_graph(vlbm_batch* b) {
     if (b->graph_exec) { (void)hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = VLBM_OK;
+    const int rheo = common_rheo(b);
     for (int k = 0; k < b->K && !rc; k++)
-        if (launch_batch_step(b->d.q, b->d.precision, b->block, (uint32_t)b->grid, b->d_jobs, b->d_prefix,
-                              b->d_njobs, k, b->stream))
+        if (launch_batch_step(b->d.q, b->d.precision, b->block, rheo, (uint32_t)b->grid, b->d_jobs, b->d_prefix,
+                              b->d_unit_job, b->d_njobs, k, b->stream))
             rc = fail(VLBM_ERR_DEVICE, "batched step launch during capture failed");
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -300,22 +337,24 @@ int build_graph(vlbm_batch* b) {
     if (e != hipSuccess) return fail(VLBM_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     b->graph = g;
     HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
-    b->graph_K = b->K; b->graph_block = b->block; b->graph_grid = b->grid;
+    b->graph_K = b->K; b->graph_block = b->block; b->graph_grid = b->grid; b->graph_rheo = rheo;
     return VLBM_OK;
 }
 
 // K steps of every running job
 int run_chunk(vlbm_batch* b) {
     if (b->d.use_graph) {
-        if (!b->graph_exec || b->graph_K != b->K || b->graph_block != b->block || b->graph_grid != b->grid) {
+        if (!b->graph_exec || b->graph_K != b->K || b->graph_block != b->block || b->graph_grid != b->grid ||
+            b->graph_rheo != common_rheo(b)) {
             const int rc = build_graph(b);
             if (rc) return rc;
         }
         HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
     } else {
+        const int rheo = common_rheo(b);
         for (int k = 0; k < b->K; k++)
-            if (launch_batch_step(b->d.q, b->d.precision, b->block, (uint32_t)b->grid, b->d_jobs, b->d_prefix,
-                                  b->d_njobs, k, b->stream))
+            if (launch_batch_step(b->d.q, b->d.precision, b->block, rheo, (uint32_t)b->grid, b->d_jobs, b->d_prefix,
+                                  b->d_unit_job, b->d_njobs, k, b->stream))
                 return check_launch(-1, "batched step");
     }
     for (int id : b->running) b->jobs[id].steps += b->K;
@@ -334,8 +373,8 @@ int check_jobs(vlbm_batch* b) {
     int rc = upload(b, due);
     if (rc) return rc;
     const uint32_t nj = (uint32_t)b->running.size();
-    if (launch_batch_check(b->d.q, b->d.precision, b->block, (uint32_t)b->grid, b->d_jobs, b->d_prefix, b->d_njobs,
-                           nj, b->d_partials, b->d_sums, b->stream))
+    if (launch_batch_check(b->d.q, b->d.precision, b->block, (uint32_t)b->grid, b->d_jobs, b->d_prefix,
+                           b->d_unit_job, b->d_njobs, nj, b->d_partials, b->d_sums, b->stream))
         return check_launch(-1, "residual check");
     std::vector<double> sums(3 * (size_t)nj);
     HIP_TRY(hipMemcpyAsync(sums.data(), b->d_sums, sums.size() * 8, hipMemcpyDeviceToHost, b->stream));
@@ -480,6 +519,7 @@ void vlbm_batch_desc_defaults(vlbm_batch_desc* d) {
     d->grid = 0;
     d->use_graph = 1;
     d->tune_cache = nullptr;
+    d->max_resident_nodes = 0;
 }
 
 int vlbm_batch_create(const vlbm_batch_desc* d, vlbm_batch_h* out) {
@@ -518,6 +558,15 @@ int vlbm_batch_create(const vlbm_batch_desc* d, vlbm_batch_h* out) {
         hipMalloc((void**)&b->d_njobs, 4) != hipSuccess ||
         hipMalloc((void**)&b->d_sums, 3 * sizeof(double) * d->max_jobs) != hipSuccess)
         return bail(VLBM_ERR_ALLOC, "job table allocation failed");
+    if (d->max_jobs > 65535) return bail(VLBM_ERR_INVALID, "max_jobs must be <= 65535");
+    {
+        // every job holds two lattices and the previous velocity, so the arena
+        // bounds the resident nodes and the 128-node work units
+        const size_t per_node = 2 * (size_t)d->q * d->precision + 3 * (size_t)b->tbytes;
+        b->unit_cap = b->arena_bytes / per_node / 128 + (size_t)d->max_jobs + 1;
+        if (hipMalloc((void**)&b->d_unit_job, 2 * b->unit_cap) != hipSuccess)
+            return bail(VLBM_ERR_ALLOC, "work-unit table allocation failed");
+    }
     b->K = d->steps_per_launch ? d->steps_per_launch : k_candidates(d->check_every).front();
     b->block = d->block ? d->block : 256;
     b->grid = d->grid ? d->grid : 2048;
@@ -536,6 +585,7 @@ void vlbm_batch_destroy(vlbm_batch_h b) {
     if (b->arena) (void)hipFree(b->arena);
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->d_prefix) (void)hipFree(b->d_prefix);
+    if (b->d_unit_job) (void)hipFree(b->d_unit_job);
     if (b->d_njobs) (void)hipFree(b->d_njobs);
     if (b->d_sums) (void)hipFree(b->d_sums);
     if (b->d_partials) (void)hipFree(b->d_partials);

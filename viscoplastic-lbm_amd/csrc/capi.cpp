@@ -201,6 +201,9 @@ int validate(const vlbm_desc* d) {
     if (d->les_const > 0 && d->rheology != VLBM_NEWTONIAN)
         return fail(VLBM_ERR_UNSUPPORTED, "LES applies to Newtonian fluids only");
     if (d->les_const > 0 && !(d->tau > 0.5)) return fail(VLBM_ERR_INVALID, "LES needs tau > 0.5");
+    if (d->layout < VLBM_LAYOUT_AUTO || d->layout > VLBM_LAYOUT_MOMENTS) return fail(VLBM_ERR_INVALID, "unknown layout");
+    if (d->layout == VLBM_LAYOUT_MOMENTS && (d->collision != VLBM_COLLISION_REGULARIZED || d->precision == VLBM_FP16))
+        return fail(VLBM_ERR_UNSUPPORTED, "the moment layout holds the regularized state in float or double storage");
     if (d->q == 27 && d->builder != VLBM_BUILDER_TAYLOR_GREEN) {
         // Zou-He and velocity bounce-back exist for D3Q19 only in the reference
         // (boundaryConditionsHandler.cu:274,318,361); refuse instead of silently skipping.
@@ -316,7 +319,14 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
         release(h);
         return fail(VLBM_ERR_DEVICE, "hipEventCreate failed");
     }
-    const size_t lat_bytes = (size_t)d->q * h->n * h->sbytes;
+    // AUTO: coefficients where they were measured faster on MI355X (D3Q27 in
+    // float storage: 28.5 vs 25.0 GLUPS at 256^3); D3Q19 runs level and FP64
+    // is faster with populations
+    const bool mom_auto = d->collision == VLBM_COLLISION_REGULARIZED && d->precision == VLBM_FP32 && d->q == 27 &&
+                          d->nx >= 64;
+    h->mom = d->layout == VLBM_LAYOUT_MOMENTS || (d->layout == VLBM_LAYOUT_AUTO && mom_auto);
+    h->planes = h->mom ? kCoef : d->q;
+    const size_t lat_bytes = (size_t)h->planes * h->n * h->sbytes;
     for (int k = 0; k < 2; k++) {
         if ((rc = dalloc(h, &h->lat[k], lat_bytes))) { release(h); return rc; }
         if (hipMemsetAsync(h->lat[k], 0, lat_bytes, h->stream) != hipSuccess) { release(h); return fail(VLBM_ERR_DEVICE, "hipMemset"); }
@@ -388,7 +398,7 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
         }
         h->cur = 0;
         if (launch_fill_feq(h->d.q, h->sbytes, h->lat[0], h->stride, (uint32_t)n, dev, dev + n, dev + 2 * n,
-                            dev + 3 * n, h->stream)) {
+                            dev + 3 * n, h->stream, h->mom ? kRepMom : kRepPop)) {
             rc = check_launch(-1, "equilibrium kernel");
             break;
         }
@@ -409,8 +419,15 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
     // power-law omega starts at 0 (src/CUDA/lbmInitialization.cu:110-112)
     if (h->omega_state && (rc = upload_field(h, h->omega_state, zero.data()))) return rc;
     h->direct = true;
+    h->direct_pops = false;
     h->steps = 0;
     return ibm_reset_tiles(h);
+}
+
+// Q-plane population buffer of a coefficient handle, allocated on first use.
+static int need_pops(vlbm_h h) {
+    if (h->pops) return VLBM_OK;
+    return dalloc(h, &h->pops, (size_t)h->d.q * h->n * h->sbytes);
 }
 
 int vlbm_set_omega(vlbm_h h, const double* omega) {
@@ -430,12 +447,15 @@ int vlbm_load_pops(vlbm_h h, const double* pop) {
         double* dstp = dev_in.data() + (size_t)i * h->n;
         for (size_t k = 0; k < h->n; k++) dstp[k] = srcp[k] - w;
     }
+    // a coefficient handle keeps the loaded populations until the first step
+    int rc = h->mom ? need_pops(h) : VLBM_OK;
+    if (rc) return rc;
+    void* target = h->mom ? h->pops : h->lat[0];
     double* dbl = nullptr;
     HIP_TRY(hipMalloc((void**)&dbl, dev_in.size() * 8));
-    int rc = VLBM_OK;
     if (hipMemcpyAsync(dbl, dev_in.data(), dev_in.size() * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess)
         rc = fail(VLBM_ERR_DEVICE, "population upload failed");
-    else if (launch_convert(h->sbytes, h->lat[0], dbl, dev_in.size(), 0, h->stream))
+    else if (launch_convert(h->sbytes, target, dbl, dev_in.size(), 0, h->stream))
         rc = check_launch(-1, "conversion kernel");
     else if (hipStreamSynchronize(h->stream) != hipSuccess)
         rc = fail(VLBM_ERR_DEVICE, "sync failed");
@@ -443,6 +463,7 @@ int vlbm_load_pops(vlbm_h h, const double* pop) {
     if (rc) return rc;
     h->cur = 0;
     h->direct = true;
+    h->direct_pops = h->mom;
     h->steps = 0;
     return ibm_reset_tiles(h);
 }
@@ -450,26 +471,30 @@ int vlbm_load_pops(vlbm_h h, const double* pop) {
 int vlbm_get_pops(vlbm_h h, double* pop) {
     if (!h || !pop) return fail(VLBM_ERR_INVALID, "handle or output NULL");
     const vlbm_desc& d = h->d;
-    void* source = h->lat[h->cur];
-    if (!h->direct) {
-        // post-stream + BC populations of the current state into the idle lattice
-        const int other = 1 - h->cur;
-        int rc = with_params(h, [&](void* pv, auto& p) {
+    const void* source = state_src(h);
+    if (!h->direct || (h->mom && !h->direct_pops)) {
+        // post-stream + BC populations of the current state into the idle
+        // lattice (population layout) or the population buffer (coefficients)
+        int rc = h->mom ? need_pops(h) : VLBM_OK;
+        if (rc) return rc;
+        void* target = h->mom ? h->pops : h->lat[1 - h->cur];
+        rc = with_params(h, [&](void* pv, auto& p) {
             p.src = h->lat[h->cur];
-            p.dst = h->lat[other];
+            p.dst = target;
             p.aux_mode = kAuxMaterialize;
-            p.direct = 0;
-            return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream), "materialize kernel");
+            p.direct = h->direct ? 1 : 0;
+            return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
+                                "materialize kernel");
         });
         if (rc) return rc;
-        source = h->lat[other];
+        source = target;
     }
     const size_t total = (size_t)d.q * h->n;
     double* dbl = nullptr;
     HIP_TRY(hipMalloc((void**)&dbl, total * 8));
     std::vector<double> host(total);
     int rc = VLBM_OK;
-    if (launch_convert(h->sbytes, source, dbl, total, 1, h->stream)) rc = check_launch(-1, "conversion kernel");
+    if (launch_convert(h->sbytes, (void*)source, dbl, total, 1, h->stream)) rc = check_launch(-1, "conversion kernel");
     else if (hipMemcpyAsync(host.data(), dbl, total * 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
              hipStreamSynchronize(h->stream) != hipSuccess)
         rc = fail(VLBM_ERR_DEVICE, "population download failed");
@@ -496,17 +521,19 @@ static int enqueue_steps(vlbm_h h, int n_steps, int save_last, hipEvent_t* ev = 
         const int other = 1 - h->cur;
         const bool save = save_last && (s == n_steps - 1);
         int rc = with_params(h, [&](void* pv, auto& p) {
-            p.src = h->lat[h->cur];
+            p.src = state_src(h);
             p.dst = h->lat[other];
             // IBM builds take the macroscopics from the stage (save_macr_to_array
             // is false with IBM, src/CUDA/main.cu:293-297)
             p.save = (save && !ibm) ? 1 : 0;
-            return check_launch(launch_step(d.q, h->sbytes, d.rheology, h->mask_geom, h->direct, pv, h->stream, ibm),
+            return check_launch(launch_step(d.q, h->sbytes, d.rheology, h->mask_geom, h->direct, pv, h->stream, ibm,
+                                            state_rep(h), h->mom ? kRepMom : kRepPop),
                                 "step kernel");
         });
         if (rc) return rc;
         h->cur = other;
         h->direct = false;
+        h->direct_pops = false;
         if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], h->stream));
         if (ibm) {
             if ((rc = ibm_stage(h))) return rc;
@@ -569,11 +596,12 @@ int vlbm_compute_macroscopics(vlbm_h h) {
     if (h->ibm) return ibm_fill_macr(h);
     const vlbm_desc& d = h->d;
     return with_params(h, [&](void* pv, auto& p) {
-        p.src = h->lat[h->cur];
+        p.src = state_src(h);
         p.dst = h->lat[1 - h->cur];
         p.aux_mode = kAuxMoments;
         p.direct = h->direct ? 1 : 0;
-        return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream), "moments kernel");
+        return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
+                            "moments kernel");
     });
 }
 

@@ -43,6 +43,11 @@ struct vlbm_ctx {
     void* lat[2] = {nullptr, nullptr};
     int cur = 0;              // lattice holding the current state
     bool direct = true;       // current state is post-stream (after init / load)
+    bool mom = false;         // lattices hold moment coefficients (kRepMom)
+    int planes = 0;           // planes per lattice: Q, or 10 coefficients
+    void* pops = nullptr;     // Q-plane population buffer of a coefficient handle:
+                              // loaded post-stream state, or scratch of vlbm_get_pops
+    bool direct_pops = false; // the current (direct) state is in pops
     long long steps = 0;
     uint32_t* d_mask = nullptr;
     std::vector<uint32_t> words;   // host map
@@ -141,9 +146,14 @@ int ibm_stage(vlbm_ctx* h);        // one stage on the current state
 int ibm_fill_macr(vlbm_ctx* h);    // macroscopics outside the flagged tiles
 int ibm_reset_tiles(vlbm_ctx* h);  // no flagged tiles: every node takes the constant force
 
+// Representation and buffer of the current state.
+inline int state_rep(const vlbm_ctx* h) { return (h->mom && !h->direct_pops) ? 1 : 0; }
+inline const void* state_src(const vlbm_ctx* h) { return h->direct_pops ? h->pops : h->lat[h->cur]; }
+
 inline void release(vlbm_ctx* h) {
     if (!h) return;
     ibm_release(h);
+    if (h->pops) (void)hipFree(h->pops);
     for (void* p : h->lat) if (p) (void)hipFree(p);
     for (void* p : h->macr) if (p) (void)hipFree(p);
     for (void* p : h->uold) if (p) (void)hipFree(p);

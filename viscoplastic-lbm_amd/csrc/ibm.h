@@ -100,7 +100,8 @@ struct IbmFluid {
 
 // Launchers (kernels_ibm.hip and the per-precision kernel files).
 int launch_ibm_macr(int q, int prec, bool mask_geom, const void* params, const void* fluid,
-                    const uint32_t* tile_list, const uint32_t* tile_count, uint32_t max_tiles, void* stream);
+                    const uint32_t* tile_list, const uint32_t* tile_count, uint32_t max_tiles, void* stream,
+                    int in_rep = 0);
 int launch_ibm_stage(const IbmParams& P, int tbytes, const void* fluid, int stage_part, void* stream);
 
 // Parts of one stage, enqueued in this order (immersedBoundaryMethod, ibm.cu:6-192).

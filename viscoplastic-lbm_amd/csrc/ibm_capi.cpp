@@ -277,12 +277,12 @@ int ibm_stage(vlbm_ctx* h) {
     const uint32_t max_tiles = s.P.n_tiles;
     rc = with_params(h, [&](void* pv, auto& p) {
         using T = typename std::remove_reference<decltype(*p.m_rho)>::type;
-        p.src = h->lat[h->cur];
+        p.src = state_src(h);
         p.dst = h->lat[1 - h->cur];
         p.direct = h->direct ? 1 : 0;
         p.m_rho = (T*)s.macr[0]; p.m_ux = (T*)s.macr[1]; p.m_uy = (T*)s.macr[2]; p.m_uz = (T*)s.macr[3];
         return check_launch(launch_ibm_macr(d.q, h->sbytes, h->mask_geom, pv, nullptr, s.P.tile_list, s.P.tile_count,
-                                            max_tiles, h->stream),
+                                            max_tiles, h->stream, state_rep(h)),
                             "IBM macroscopics kernel");
     });
     if (rc) return rc;
@@ -297,11 +297,12 @@ int ibm_stage(vlbm_ctx* h) {
 int ibm_fill_macr(vlbm_ctx* h) {
     const vlbm_desc& d = h->d;
     int rc = with_params(h, [&](void* pv, auto& p) {
-        p.src = h->lat[h->cur];
+        p.src = state_src(h);
         p.dst = h->lat[1 - h->cur];
         p.aux_mode = kAuxMomentsOutsideTiles;
         p.direct = h->direct ? 1 : 0;
-        return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream), "moments kernel");
+        return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
+                            "moments kernel");
     });
     if (rc) return rc;
     return stage_part(h, kIbmOutput);

@@ -6,46 +6,47 @@
 
 namespace vlbm {
 
-// job owning work unit w: prefix[j] <= w < prefix[j+1]
-__device__ __forceinline__ uint32_t job_of(const uint32_t* prefix, uint32_t nj, uint32_t w) {
-    uint32_t lo = 0, hi = nj;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (prefix[mid] <= w) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
+// Rheology of every resident job when they share one (the kernel is then
+// specialised to it); kMixedRheo switches per work unit.
+constexpr int kMixedRheo = 3;
 
-template <int Q, typename S, int BLOCK>
+template <int Q, typename S, int BLOCK, int RHEO>
 __global__ __launch_bounds__(BLOCK) void k_batch_step(const BatchJob<typename ComputeOf<S>::type>* __restrict__ jobs,
                                                       const uint32_t* __restrict__ prefix,
+                                                      const uint16_t* __restrict__ unit_job,
                                                       const uint32_t* __restrict__ n_jobs, int k) {
     using T = typename ComputeOf<S>::type;
     const uint32_t nj = *n_jobs;
     const uint32_t total = nj ? prefix[nj] : 0u;
     for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
-        const uint32_t j = job_of(prefix, nj, w);
+        const uint32_t j = unit_job[w];
         const BatchJob<T>& J = jobs[j];
         const Params<T>& p = J.p[(J.phase + k) & 1];
         const uint32_t idx = (w - prefix[j]) * BLOCK + threadIdx.x;
         if (idx >= p.n) continue;
-        if (J.rheo == 0) step_node<Q, S, 0, false, false, false>(p, idx);
-        else if (J.rheo == 1) step_node<Q, S, 1, false, false, false>(p, idx);
-        else step_node<Q, S, 2, false, false, false>(p, idx);
+        if (RHEO != kMixedRheo) {
+            step_node<Q, S, (RHEO == kMixedRheo ? 0 : RHEO), false, false, false>(p, idx);
+        } else if (J.rheo == 0) {
+            step_node<Q, S, 0, false, false, false>(p, idx);
+        } else if (J.rheo == 1) {
+            step_node<Q, S, 1, false, false, false>(p, idx);
+        } else {
+            step_node<Q, S, 2, false, false, false>(p, idx);
+        }
     }
 }
 
 template <int Q, typename S, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch_check(const BatchJob<typename ComputeOf<S>::type>* __restrict__ jobs,
                                                        const uint32_t* __restrict__ prefix,
+                                                       const uint16_t* __restrict__ unit_job,
                                                        const uint32_t* __restrict__ n_jobs, double* partials) {
     using T = typename ComputeOf<S>::type;
     __shared__ double sh[3][BLOCK / 64];
     const uint32_t nj = *n_jobs;
     const uint32_t total = nj ? prefix[nj] : 0u;
     for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
-        const uint32_t j = job_of(prefix, nj, w);
+        const uint32_t j = unit_job[w];
         const BatchJob<T>& J = jobs[j];
         if (!J.due) continue;                       // uniform per block
         const Params<T>& p = J.p[J.phase];          // current state = source of the next step
@@ -54,9 +55,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_check(const BatchJob<typename C
         if (idx < p.n) {
             const Node<T> nd = make_node(p, idx);
             const uint32_t word = node_word<false>(p, nd);
+            const GlobalPC<Q, S, T, kRepPop> pc{p, nd};
             T g[Q];
-            gather<Q, S, T>(g, p, nd);
-            if (w_used(word) && w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
+            gather<Q, T>(g, pc);
+            if (w_used(word) && w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
             const Moments<Q, 0, T> m = moments<Q, 0, T>(g, p, (T)0, const_force(p));
             const double ux = (double)m.ux, uy = (double)m.uy, uz = (double)m.uz;
             const double dx = ux - (double)J.uold[0][idx], dy = uy - (double)J.uold[1][idx], dz = uz - (double)J.uold[2][idx];
@@ -112,53 +114,71 @@ __global__ __launch_bounds__(kBlock) void k_batch_fill_rest(S* lat, uint32_t n, 
     for (int i = 0; i < Q; i++) lat[(int64_t)i * n + idx] = (S)((T)weight<Q>(i) * ((T)rho0 - (T)1));
 }
 
-template <int Q, typename S>
-static int step_s(int block, uint32_t grid, const void* jobs, const uint32_t* prefix, const uint32_t* nj, int k,
-                  hipStream_t st) {
+template <int Q, typename S, int BLOCK>
+static int step_r(int rheo, uint32_t grid, const void* jobs, const uint32_t* prefix, const uint16_t* unit_job,
+                  const uint32_t* nj, int k, hipStream_t st) {
     using T = typename ComputeOf<S>::type;
     const BatchJob<T>* J = (const BatchJob<T>*)jobs;
-    if (block == 128) hipLaunchKernelGGL((k_batch_step<Q, S, 128>), dim3(grid), dim3(128), 0, st, J, prefix, nj, k);
-    else hipLaunchKernelGGL((k_batch_step<Q, S, 256>), dim3(grid), dim3(256), 0, st, J, prefix, nj, k);
+    const dim3 g(grid), b(BLOCK);
+    if (rheo == 0) hipLaunchKernelGGL((k_batch_step<Q, S, BLOCK, 0>), g, b, 0, st, J, prefix, unit_job, nj, k);
+    else if (rheo == 1) hipLaunchKernelGGL((k_batch_step<Q, S, BLOCK, 1>), g, b, 0, st, J, prefix, unit_job, nj, k);
+    else if (rheo == 2) hipLaunchKernelGGL((k_batch_step<Q, S, BLOCK, 2>), g, b, 0, st, J, prefix, unit_job, nj, k);
+    else hipLaunchKernelGGL((k_batch_step<Q, S, BLOCK, kMixedRheo>), g, b, 0, st, J, prefix, unit_job, nj, k);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int Q, typename S>
-static int check_s(int block, uint32_t grid, const void* jobs, const uint32_t* prefix, const uint32_t* nj,
-                   uint32_t nj_host, double* partials, double* sums, hipStream_t st) {
+static int step_s(int block, int rheo, uint32_t grid, const void* jobs, const uint32_t* prefix,
+                  const uint16_t* unit_job, const uint32_t* nj, int k, hipStream_t st) {
+    if (block == 128) return step_r<Q, S, 128>(rheo, grid, jobs, prefix, unit_job, nj, k, st);
+    return step_r<Q, S, 256>(rheo, grid, jobs, prefix, unit_job, nj, k, st);
+}
+
+template <int Q, typename S>
+static int check_s(int block, uint32_t grid, const void* jobs, const uint32_t* prefix, const uint16_t* unit_job,
+                   const uint32_t* nj, uint32_t nj_host, double* partials, double* sums, hipStream_t st) {
     using T = typename ComputeOf<S>::type;
     const BatchJob<T>* J = (const BatchJob<T>*)jobs;
-    if (block == 128) hipLaunchKernelGGL((k_batch_check<Q, S, 128>), dim3(grid), dim3(128), 0, st, J, prefix, nj, partials);
-    else hipLaunchKernelGGL((k_batch_check<Q, S, 256>), dim3(grid), dim3(256), 0, st, J, prefix, nj, partials);
+    if (block == 128)
+        hipLaunchKernelGGL((k_batch_check<Q, S, 128>), dim3(grid), dim3(128), 0, st, J, prefix, unit_job, nj, partials);
+    else
+        hipLaunchKernelGGL((k_batch_check<Q, S, 256>), dim3(grid), dim3(256), 0, st, J, prefix, unit_job, nj, partials);
     if (nj_host) hipLaunchKernelGGL((k_batch_job_sums<T>), dim3(nj_host), dim3(64), 0, st, J, prefix, partials, sums);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_batch_step(int q, int prec, int block, uint32_t grid, const void* jobs, const uint32_t* prefix,
-                      const uint32_t* n_jobs, int k, void* stream) {
+int launch_batch_step(int q, int prec, int block, int rheo, uint32_t grid, const void* jobs, const uint32_t* prefix,
+                      const uint16_t* unit_job, const uint32_t* n_jobs, int k, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     if (block != 128 && block != 256) return -1;
+    if (rheo < 0 || rheo > kMixedRheo) return -1;
+#define VLBM_STEP(QQ, SS) step_s<QQ, SS>(block, rheo, grid, jobs, prefix, unit_job, n_jobs, k, st)
     if (q == 19) {
-        if (prec == 8) return step_s<19, double>(block, grid, jobs, prefix, n_jobs, k, st);
-        if (prec == 4) return step_s<19, float>(block, grid, jobs, prefix, n_jobs, k, st);
-        return step_s<19, _Float16>(block, grid, jobs, prefix, n_jobs, k, st);
+        if (prec == 8) return VLBM_STEP(19, double);
+        if (prec == 4) return VLBM_STEP(19, float);
+        return VLBM_STEP(19, _Float16);
     }
-    if (prec == 8) return step_s<27, double>(block, grid, jobs, prefix, n_jobs, k, st);
-    if (prec == 4) return step_s<27, float>(block, grid, jobs, prefix, n_jobs, k, st);
-    return step_s<27, _Float16>(block, grid, jobs, prefix, n_jobs, k, st);
+    if (prec == 8) return VLBM_STEP(27, double);
+    if (prec == 4) return VLBM_STEP(27, float);
+    return VLBM_STEP(27, _Float16);
+#undef VLBM_STEP
 }
 
 int launch_batch_check(int q, int prec, int block, uint32_t grid, const void* jobs, const uint32_t* prefix,
-                       const uint32_t* n_jobs, uint32_t n_jobs_host, double* partials, double* sums, void* stream) {
+                       const uint16_t* unit_job, const uint32_t* n_jobs, uint32_t n_jobs_host, double* partials,
+                       double* sums, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     if (block != 128 && block != 256) return -1;
+#define VLBM_CHECK(QQ, SS) check_s<QQ, SS>(block, grid, jobs, prefix, unit_job, n_jobs, n_jobs_host, partials, sums, st)
     if (q == 19) {
-        if (prec == 8) return check_s<19, double>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
-        if (prec == 4) return check_s<19, float>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
-        return check_s<19, _Float16>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
+        if (prec == 8) return VLBM_CHECK(19, double);
+        if (prec == 4) return VLBM_CHECK(19, float);
+        return VLBM_CHECK(19, _Float16);
     }
-    if (prec == 8) return check_s<27, double>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
-    if (prec == 4) return check_s<27, float>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
-    return check_s<27, _Float16>(block, grid, jobs, prefix, n_jobs, n_jobs_host, partials, sums, st);
+    if (prec == 8) return VLBM_CHECK(27, double);
+    if (prec == 4) return VLBM_CHECK(27, float);
+    return VLBM_CHECK(27, _Float16);
+#undef VLBM_CHECK
 }
 
 int launch_batch_fill_rest(int q, int prec, void* lat, uint32_t n, double rho0, void* stream) {

@@ -4,18 +4,21 @@
 
 namespace vlbm {
 
-int launch_step_fp64(int q, int rheo, bool mask, bool direct, bool ibm, const void* p, void* stream);
-int launch_step_fp32(int q, int rheo, bool mask, bool direct, bool ibm, const void* p, void* stream);
-int launch_step_fp16(int q, int rheo, bool mask, bool direct, bool ibm, const void* p, void* stream);
-int launch_ibm_macr_fp64(int q, bool mask, const void* p, const uint32_t* list, const uint32_t* count,
+int launch_step_fp64(int q, int rheo, bool mask, bool direct, bool ibm, int in_rep, int out_rep, const void* p,
+                     void* stream);
+int launch_step_fp32(int q, int rheo, bool mask, bool direct, bool ibm, int in_rep, int out_rep, const void* p,
+                     void* stream);
+int launch_step_fp16(int q, int rheo, bool mask, bool direct, bool ibm, int in_rep, int out_rep, const void* p,
+                     void* stream);
+int launch_ibm_macr_fp64(int q, bool mask, int in_rep, const void* p, const uint32_t* list, const uint32_t* count,
                          uint32_t max_tiles, void* stream);
-int launch_ibm_macr_fp32(int q, bool mask, const void* p, const uint32_t* list, const uint32_t* count,
+int launch_ibm_macr_fp32(int q, bool mask, int in_rep, const void* p, const uint32_t* list, const uint32_t* count,
                          uint32_t max_tiles, void* stream);
-int launch_ibm_macr_fp16(int q, bool mask, const void* p, const uint32_t* list, const uint32_t* count,
+int launch_ibm_macr_fp16(int q, bool mask, int in_rep, const void* p, const uint32_t* list, const uint32_t* count,
                          uint32_t max_tiles, void* stream);
-int launch_aux_fp64(int q, int rheo, bool mask, const void* p, void* stream);
-int launch_aux_fp32(int q, int rheo, bool mask, const void* p, void* stream);
-int launch_aux_fp16(int q, int rheo, bool mask, const void* p, void* stream);
+int launch_aux_fp64(int q, int rheo, bool mask, int in_rep, const void* p, void* stream);
+int launch_aux_fp32(int q, int rheo, bool mask, int in_rep, const void* p, void* stream);
+int launch_aux_fp16(int q, int rheo, bool mask, int in_rep, const void* p, void* stream);
 
 __global__ void k_reduce_final(const double* partials, int nblocks, double* out3) {
     // one wave; fixed order -> deterministic
@@ -33,34 +36,40 @@ __global__ void k_reduce_final(const double* partials, int nblocks, double* out3
 
 static inline dim3 grid_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
-int launch_step(int q, int prec, int rheo, bool mask, bool direct, const void* p, void* stream, bool ibm) {
-    if (prec == 8) return launch_step_fp64(q, rheo, mask, direct, ibm, p, stream);
-    if (prec == 4) return launch_step_fp32(q, rheo, mask, direct, ibm, p, stream);
-    return launch_step_fp16(q, rheo, mask, direct, ibm, p, stream);
+int launch_step(int q, int prec, int rheo, bool mask, bool direct, const void* p, void* stream, bool ibm, int in_rep,
+                int out_rep) {
+    if (prec == 8) return launch_step_fp64(q, rheo, mask, direct, ibm, in_rep, out_rep, p, stream);
+    if (prec == 4) return launch_step_fp32(q, rheo, mask, direct, ibm, in_rep, out_rep, p, stream);
+    if (in_rep != kRepPop || out_rep != kRepPop) return -1;
+    return launch_step_fp16(q, rheo, mask, direct, ibm, in_rep, out_rep, p, stream);
 }
 
 int launch_ibm_macr(int q, int prec, bool mask, const void* p, const void* /*fluid*/, const uint32_t* list,
-                    const uint32_t* count, uint32_t max_tiles, void* stream) {
-    if (prec == 8) return launch_ibm_macr_fp64(q, mask, p, list, count, max_tiles, stream);
-    if (prec == 4) return launch_ibm_macr_fp32(q, mask, p, list, count, max_tiles, stream);
-    return launch_ibm_macr_fp16(q, mask, p, list, count, max_tiles, stream);
+                    const uint32_t* count, uint32_t max_tiles, void* stream, int in_rep) {
+    if (prec == 8) return launch_ibm_macr_fp64(q, mask, in_rep, p, list, count, max_tiles, stream);
+    if (prec == 4) return launch_ibm_macr_fp32(q, mask, in_rep, p, list, count, max_tiles, stream);
+    return launch_ibm_macr_fp16(q, mask, in_rep, p, list, count, max_tiles, stream);
 }
 
-int launch_aux(int q, int prec, int rheo, bool mask, const void* p, void* stream) {
-    if (prec == 8) return launch_aux_fp64(q, rheo, mask, p, stream);
-    if (prec == 4) return launch_aux_fp32(q, rheo, mask, p, stream);
-    return launch_aux_fp16(q, rheo, mask, p, stream);
+int launch_aux(int q, int prec, int rheo, bool mask, const void* p, void* stream, int in_rep) {
+    if (prec == 8) return launch_aux_fp64(q, rheo, mask, in_rep, p, stream);
+    if (prec == 4) return launch_aux_fp32(q, rheo, mask, in_rep, p, stream);
+    return launch_aux_fp16(q, rheo, mask, in_rep, p, stream);
 }
 
 int launch_fill_feq(int q, int prec, void* state, int64_t stride, uint32_t n, const double* rho,
-                    const double* ux, const double* uy, const double* uz, void* stream) {
+                    const double* ux, const double* uy, const double* uz, void* stream, int out_rep) {
     hipStream_t st = (hipStream_t)stream;
     const dim3 grid = grid_for(n);
-#define VLBM_FILL(QQ, SS) hipLaunchKernelGGL((k_fill_feq<QQ, SS>), grid, dim3(kBlock), 0, st, (SS*)state, stride, n, rho, ux, uy, uz)
-    if (q == 19) {
-        if (prec == 8) VLBM_FILL(19, double); else if (prec == 4) VLBM_FILL(19, float); else VLBM_FILL(19, _Float16);
+#define VLBM_FILL(QQ, SS, RR) hipLaunchKernelGGL((k_fill_feq<QQ, SS, RR>), grid, dim3(kBlock), 0, st, (SS*)state, stride, n, rho, ux, uy, uz)
+    if (out_rep == kRepMom) {
+        if (prec == 8) VLBM_FILL(19, double, kRepMom);
+        else if (prec == 4) VLBM_FILL(19, float, kRepMom);
+        else return -1;
+    } else if (q == 19) {
+        if (prec == 8) VLBM_FILL(19, double, kRepPop); else if (prec == 4) VLBM_FILL(19, float, kRepPop); else VLBM_FILL(19, _Float16, kRepPop);
     } else {
-        if (prec == 8) VLBM_FILL(27, double); else if (prec == 4) VLBM_FILL(27, float); else VLBM_FILL(27, _Float16);
+        if (prec == 8) VLBM_FILL(27, double, kRepPop); else if (prec == 4) VLBM_FILL(27, float, kRepPop); else VLBM_FILL(27, _Float16, kRepPop);
     }
 #undef VLBM_FILL
     return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -248,8 +248,9 @@ __device__ T wall_q(int kind, T x1, T y1, T x2, T y2, T R) {
     }
 }
 
-template <int Q, typename S, typename T>
-__device__ void interp_bounce_back(T* g, const Params<T>& p, const Node<T>& nd, unsigned unknown, bool inside) {
+template <int Q, typename T, class PC>
+__device__ void interp_bounce_back(T* g, const Params<T>& p, const Node<T>& nd, unsigned unknown, bool inside,
+                                   PC pc) {
     // D2Q9 projections in the reference's bit order (nodeTypeMap.h:100-107)
     const int dxs[8] = {1, 0, -1, 0, 1, -1, -1, 1};
     const int dys[8] = {0, 1, 0, -1, 1, 1, -1, -1};
@@ -264,7 +265,6 @@ __device__ void interp_bounce_back(T* g, const Params<T>& p, const Node<T>& nd, 
         const T wo = p.w_out * ((T)nd.z / (T)p.nz);
         radius = p.duct_R; uxw = -wo * radius * s; uyw = wo * radius * c; uzw = p.u_out;
     }
-    const uint32_t self = nd.idx;
     for (int b = 0; b < 8; b++) {
         if (!(unknown & (1u << b))) continue;
         const T qd = wall_q<T>(kinds[b], xN, yN, xN - (T)dxs[b], yN - (T)dys[b], radius);
@@ -274,17 +274,14 @@ __device__ void interp_bounce_back(T* g, const Params<T>& p, const Node<T>& nd, 
             const int j = opp(i);
             const T cu = (T)kCX[i] * uxw + (T)kCY[i] * uyw + (T)kCZ[i] * uzw;
             const T df = (T)weight<Q>(i) * cu;
-            const T fj = load_pop<S, T>(p, j, self);
+            const T fj = pc(j, 0, 0, 0);
             if (qd > (T)0.5) {
-                const T fi = load_pop<S, T>(p, i, self);
+                const T fi = pc(i, 0, 0, 0);
                 g[i] = ((T)0.5 / qd) * fj + ((qd - (T)0.5) / qd) * fi + (T)3 * df / qd;
             } else {
                 // the reference reads x + c_i unwrapped; wrapping keeps degenerate
                 // (1-2 node wide) lattices in bounds and is the same node otherwise
-                const int xa = kCX[i] > 0 ? nd.xp : (kCX[i] < 0 ? nd.xm : nd.x);
-                const int ya = kCY[i] > 0 ? nd.yp : (kCY[i] < 0 ? nd.ym : nd.y);
-                const int za = kCZ[i] > 0 ? nd.zp : (kCZ[i] < 0 ? nd.zm : nd.z);
-                const T fa = load_pop<S, T>(p, j, row_off(p, ya, za) + (uint32_t)xa);
+                const T fa = pc(j, kCX[i], kCY[i], kCZ[i]);
                 g[i] = 2 * qd * fj + (1 - 2 * qd) * fa + (T)6 * df;
             }
         }
@@ -318,23 +315,70 @@ __device__ __forceinline__ uint32_t node_word(const Params<T>& p, const Node<T>&
     return p.region_word[box_region(nd.x, nd.y, nd.z, p.nx, p.ny, p.nz)];
 }
 
-// Gather (pull streaming) of the post-stream deviations.
-template <int Q, typename S, typename T>
-__device__ __forceinline__ void gather(T* g, const Params<T>& p, const Node<T>& nd) {
-#pragma unroll
-    for (int i = 0; i < Q; i++) g[i] = load_pop<S, T>(p, i, src_index(p, nd, i));
+// ---------------------------------------------------------------------------
+// State representations. kRepPop: post-collision deviations g*_i, Q planes.
+// kRepMom: the regularized post-collision state as 10 coefficients per node,
+//   g*_i = w_i (A + B.c_i + sum_a c_ia^2 G_aa + sum_{a<b} c_ia c_ib G_ab),
+// planes in the order A, Bx, By, Bz, Gxx, Gyy, Gzz, Gxy, Gxz, Gyz. The
+// regularized collision (and the equilibrium) is a quadratic polynomial in
+// c_i, so the coefficients hold the post-collision populations exactly.
+// ---------------------------------------------------------------------------
+
+// g*_i of the node from its 10 coefficients at c (element stride st).
+template <int Q, typename T, typename P>
+__device__ __forceinline__ T recon(int i, const P* c, int64_t st) {
+    const int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+    T v = (T)c[0];
+    if (cx) v += (T)cx * (T)c[1 * st] + (T)c[4 * st];
+    if (cy) v += (T)cy * (T)c[2 * st] + (T)c[5 * st];
+    if (cz) v += (T)cz * (T)c[3 * st] + (T)c[6 * st];
+    if (cx * cy) v += (T)(cx * cy) * (T)c[7 * st];
+    if (cx * cz) v += (T)(cx * cz) * (T)c[8 * st];
+    if (cy * cz) v += (T)(cy * cz) * (T)c[9 * st];
+    return (T)weight<Q>(i) * v;
 }
 
-template <int Q, typename S, typename T>
-__device__ __forceinline__ void load_direct(T* g, const Params<T>& p, const Node<T>& nd) {
+// Node at offset (dx, dy, dz) in {-1, 0, 1}^3 from nd, wrapped.
+template <typename T>
+__device__ __forceinline__ uint32_t node_at(const Params<T>& p, const Node<T>& nd, int dx, int dy, int dz) {
+    const int x = dx > 0 ? nd.xp : (dx < 0 ? nd.xm : nd.x);
+    const int y = dy > 0 ? nd.yp : (dy < 0 ? nd.ym : nd.y);
+    const int z = dz > 0 ? nd.zp : (dz < 0 ? nd.zm : nd.z);
+    return row_off(p, y, z) + (uint32_t)x;
+}
+
+// Post-collision population i of the node at offset (dx, dy, dz), read from
+// the source lattice in global memory.
+template <int Q, typename S, typename T, int IN>
+struct GlobalPC {
+    const Params<T>& p;
+    Node<T> nd;
+    __device__ __forceinline__ T operator()(int i, int dx, int dy, int dz) const {
+        const uint32_t node = node_at(p, nd, dx, dy, dz);
+        if (IN == kRepPop) return load_pop<S, T>(p, i, node);
+        return recon<Q, T, S>(i, (const S*)p.src + node, p.stride);
+    }
+};
+
+// Gather (pull streaming) of the post-stream deviations: g_i(x) = g*_i(x - c_i).
+template <int Q, typename T, class PC>
+__device__ __forceinline__ void gather(T* g, PC pc) {
 #pragma unroll
-    for (int i = 0; i < Q; i++) g[i] = load_pop<S, T>(p, i, nd.idx);
+    for (int i = 0; i < Q; i++) g[i] = pc(i, -kCX[i], -kCY[i], -kCZ[i]);
+}
+
+// Post-stream state read in place (after an initialisation or a load).
+template <int Q, typename T, class PC>
+__device__ __forceinline__ void load_direct(T* g, PC pc) {
+#pragma unroll
+    for (int i = 0; i < Q; i++) g[i] = pc(i, 0, 0, 0);
 }
 
 // Boundary condition of the node's word on the gathered populations
-// (dispatch of src/CUDA/boundaryConditionsHandler.cu:25-79).
-template <int Q, typename S, typename T>
-__device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>& nd, uint32_t word) {
+// (dispatch of src/CUDA/boundaryConditionsHandler.cu:25-79); pc gives the
+// post-collision populations of the node and its neighbours.
+template <int Q, typename T, class PC>
+__device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>& nd, uint32_t word, PC pc) {
     const int scheme = w_scheme(word);
     const int dir = w_dir(word);
     if (scheme == kBounceBack) {
@@ -344,7 +388,7 @@ __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>
         const uint32_t m = (Q == 19) ? c_bb19.m[dir] : c_bb27.m[dir];
 #pragma unroll
         for (int i = 1; i < Q; i++)
-            if (m & (1u << i)) g[i] = load_pop<S, T>(p, opp(i), nd.idx);
+            if (m & (1u << i)) g[i] = pc(opp(i), 0, 0, 0);
     } else if (scheme == kFreeSlip) {
         // specular reflection: incoming i comes from the mirror population m
         // of the node displaced by minus the tangential part of c_i
@@ -354,23 +398,21 @@ __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>
 #pragma unroll
         for (int i = 1; i < Q; i++) {
             if (!incoming(i, faces)) continue;
+            // the source node is displaced by minus the tangential part of c_i
             int mx = kCX[i], my = kCY[i], mz = kCZ[i];
-            int sx = nd.x, sy = nd.y, sz = nd.z;
+            int ox = -mx, oy = -my, oz = -mz;
             if (dir == dN || dir == dS) {
                 my = -my;
-                sx = mx > 0 ? nd.xm : (mx < 0 ? nd.xp : nd.x);
-                sz = mz > 0 ? nd.zm : (mz < 0 ? nd.zp : nd.z);
+                oy = 0;
             } else if (dir == dW || dir == dE) {
                 mx = -mx;
-                sy = my > 0 ? nd.ym : (my < 0 ? nd.yp : nd.y);
-                sz = mz > 0 ? nd.zm : (mz < 0 ? nd.zp : nd.z);
+                ox = 0;
             } else {
                 mz = -mz;
-                sx = mx > 0 ? nd.xm : (mx < 0 ? nd.xp : nd.x);
-                sy = my > 0 ? nd.ym : (my < 0 ? nd.yp : nd.y);
+                oz = 0;
             }
             const int m = find_pop(mx, my, mz);
-            g[i] = load_pop<S, T>(p, m, row_off(p, sy, sz) + (uint32_t)sx);
+            g[i] = pc(m, ox, oy, oz);
         }
     } else if (scheme == kVelBB) {
         if (Q == 19)
@@ -380,7 +422,7 @@ __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>
             zou_he<T>(g, dir, scheme == kPresZouHe, p.ux_bc[w_uxi(word)], p.uy_bc[w_uyi(word)],
                       p.uz_bc[w_uzi(word)], p.rho_bc[w_rhoi(word)], !p.zouhe_textbook);
     } else if (scheme == kInterpBB) {
-        interp_bounce_back<Q, S, T>(g, p, nd, w_interp(word), dir == dS);
+        interp_bounce_back<Q, T>(g, p, nd, w_interp(word), dir == dS, pc);
     }
 }
 
@@ -537,28 +579,48 @@ __device__ __forceinline__ void collide_bgk(T* g, const Moments<Q, RHEO, T>& m, 
     }
 }
 
+// Regularized collision in coefficient form (kRepMom): the same g*_i as
+// collide(), as A + B.c + c^T G c with
+//   A    = (rho-1) - 1.5 rho u.u - a tr(Pi)/3 - 3 b u.F
+//   B_a  = 3 rho u_a + (3 b - a/3) F_a
+//   G_aa = 4.5 rho u_a^2 + 9 b u_a F_a + a Pi_aa
+//   G_ab = 9 rho u_a u_b + 9 b (u_a F_b + u_b F_a) + 2 a Pi_ab      (a < b)
+// with a = 4.5 (1 - omega), b = 1 - omega/2.
+template <int Q, int RHEO, typename T>
+__device__ __forceinline__ void coeffs_collide(T* c, const Moments<Q, RHEO, T>& m, const Force<T>& F) {
+    const T om = m.omega;
+    const T a = (T)4.5 * ((T)1 - om);
+    const T b = (T)1 - (T)0.5 * om;
+    const T uf = m.ux * F.x + m.uy * F.y + m.uz * F.z;
+    const T usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
+    const T tr = m.pxx + m.pyy + m.pzz;
+    const T r3 = (T)3 * m.rho, r45 = (T)4.5 * m.rho, r9 = (T)9 * m.rho;
+    const T bf = (T)3 * b - a / (T)3, b9 = (T)9 * b, a2 = (T)2 * a;
+    c[0] = m.dr - (T)1.5 * m.rho * usq - a * tr / (T)3 - (T)3 * b * uf;
+    c[1] = r3 * m.ux + bf * F.x;
+    c[2] = r3 * m.uy + bf * F.y;
+    c[3] = r3 * m.uz + bf * F.z;
+    c[4] = r45 * m.ux * m.ux + b9 * m.ux * F.x + a * m.pxx;
+    c[5] = r45 * m.uy * m.uy + b9 * m.uy * F.y + a * m.pyy;
+    c[6] = r45 * m.uz * m.uz + b9 * m.uz * F.z + a * m.pzz;
+    c[7] = r9 * m.ux * m.uy + b9 * (m.ux * F.y + m.uy * F.x) + a2 * m.pxy;
+    c[8] = r9 * m.ux * m.uz + b9 * (m.ux * F.z + m.uz * F.x) + a2 * m.pxz;
+    c[9] = r9 * m.uy * m.uz + b9 * (m.uy * F.z + m.uz * F.y) + a2 * m.pyz;
+}
+
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
 
-// One node of one time step. DIRECT: the source holds post-stream
-// populations (first step after init / load), read in place without
-// streaming or BC. IBM: the force of nodes in flagged tiles comes from the
-// IBM force field (src/CUDA/lbm.cu:45-58 reads macr.f per node in IBM builds).
-template <int Q, typename S, int RHEO, bool MASK, bool DIRECT, bool IBM>
-__device__ __forceinline__ void step_node(const Params<typename ComputeOf<S>::type>& p, uint32_t idx) {
+// Moments, apparent omega, saved macroscopics and collision of one node whose
+// post-stream, post-BC deviations are g; the post-collision state goes to
+// dst in representation OUT.
+template <int Q, typename S, int RHEO, bool IBM, int OUT>
+__device__ __forceinline__ void collide_store(const Params<typename ComputeOf<S>::type>& p, uint32_t idx,
+                                              const Node<typename ComputeOf<S>::type>& nd,
+                                              typename ComputeOf<S>::type* g) {
     using T = typename ComputeOf<S>::type;
-    const Node<T> nd = make_node(p, idx);
-    const uint32_t word = node_word<MASK>(p, nd);
-    if (!w_used(word)) return;
-    T g[Q];
-    if (DIRECT) {
-        load_direct<Q, S, T>(g, p, nd);
-    } else {
-        gather<Q, S, T>(g, p, nd);
-        if (w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
-    }
     Force<T> F = const_force(p);
     if (IBM && p.tile_flag[tile_of(p, nd.x, nd.y, nd.z)]) F = Force<T>{p.ibm_fx[idx], p.ibm_fy[idx], p.ibm_fz[idx]};
     const T om_old = (RHEO == 2) ? p.omega_state[idx] : (T)0;
@@ -568,25 +630,55 @@ __device__ __forceinline__ void step_node(const Params<typename ComputeOf<S>::ty
         p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
         if (RHEO != 0) p.m_omega[idx] = m.omega;
     }
+    S* dst = (S*)p.dst;
+    if (OUT == kRepMom) {
+        T c[kCoef];
+        coeffs_collide<Q, RHEO, T>(c, m, F);
+#pragma unroll
+        for (int k = 0; k < kCoef; k++) st_nt<S, T>(dst + (int64_t)k * p.stride + idx, c[k]);
+        return;
+    }
     if (p.bgk) collide_bgk<Q, RHEO, T>(g, m, F);
     else collide<Q, RHEO, T>(g, m, F);
-    S* dst = (S*)p.dst;
 #pragma unroll
     for (int i = 0; i < Q; i++) st_nt<S, T>(dst + (int64_t)i * p.stride + idx, g[i]);
 }
 
+// One node of one time step. DIRECT: the source holds the post-stream state
+// (first step after init / load), read in place without streaming or BC.
+// IBM: the force of nodes in flagged tiles comes from the IBM force field
+// (src/CUDA/lbm.cu:45-58 reads macr.f per node in IBM builds). IN / OUT:
+// representation of the source and destination lattices.
+template <int Q, typename S, int RHEO, bool MASK, bool DIRECT, bool IBM, int IN = kRepPop, int OUT = kRepPop>
+__device__ __forceinline__ void step_node(const Params<typename ComputeOf<S>::type>& p, uint32_t idx) {
+    using T = typename ComputeOf<S>::type;
+    const Node<T> nd = make_node(p, idx);
+    const uint32_t word = node_word<MASK>(p, nd);
+    if (!w_used(word)) return;
+    const GlobalPC<Q, S, T, IN> pc{p, nd};
+    T g[Q];
+    if (DIRECT) {
+        load_direct<Q, T>(g, pc);
+    } else {
+        gather<Q, T>(g, pc);
+        if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
+    }
+    collide_store<Q, S, RHEO, IBM, OUT>(p, idx, nd, g);
+}
+
 // One time step over the lattice, one node per thread.
-template <int Q, typename S, int RHEO, bool MASK, bool DIRECT, bool IBM = false>
+template <int Q, typename S, int RHEO, bool MASK, bool DIRECT, bool IBM = false, int IN = kRepPop, int OUT = kRepPop>
 __global__ __launch_bounds__(kBlock) void k_step(const Params<typename ComputeOf<S>::type> p) {
     const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
     if (idx >= p.n) return;
-    step_node<Q, S, RHEO, MASK, DIRECT, IBM>(p, idx);
+    step_node<Q, S, RHEO, MASK, DIRECT, IBM, IN, OUT>(p, idx);
 }
 
-// Auxiliary pass over the current state without advancing it:
+// Auxiliary pass over the current state (representation IN) without
+// advancing it:
 //   kAuxMoments:     macroscopics the next step would compute
-//   kAuxMaterialize: post-stream, post-BC populations into dst
-template <int Q, typename S, int RHEO, bool MASK>
+//   kAuxMaterialize: post-stream, post-BC populations into dst (Q planes)
+template <int Q, typename S, int RHEO, bool MASK, int IN = kRepPop>
 __global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<S>::type> p) {
     using T = typename ComputeOf<S>::type;
     const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
@@ -594,12 +686,13 @@ __global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<
     const Node<T> nd = make_node(p, idx);
     if (p.aux_mode == kAuxMomentsOutsideTiles && p.tile_flag[tile_of(p, nd.x, nd.y, nd.z)]) return;
     const uint32_t word = node_word<MASK>(p, nd);
+    const GlobalPC<Q, S, T, IN> pc{p, nd};
     T g[Q];
     if (p.direct || !w_used(word)) {
-        load_direct<Q, S, T>(g, p, nd);
+        load_direct<Q, T>(g, pc);
     } else {
-        gather<Q, S, T>(g, p, nd);
-        if (w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
+        gather<Q, T>(g, pc);
+        if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
     }
     if (p.aux_mode == kAuxMaterialize) {
         S* dst = (S*)p.dst;
@@ -618,7 +711,7 @@ __global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<
 // rho and u of the post-step populations with the constant force; the force
 // field of those nodes restarts from the constant force. Grid-stride over the
 // tile list (its length is read on the device).
-template <int Q, typename S, bool MASK>
+template <int Q, typename S, bool MASK, int IN = kRepPop>
 __global__ __launch_bounds__(kBlock) void k_ibm_macr(const Params<typename ComputeOf<S>::type> p,
                                                      const uint32_t* tile_list, const uint32_t* tile_count) {
     using T = typename ComputeOf<S>::type;
@@ -639,12 +732,13 @@ __global__ __launch_bounds__(kBlock) void k_ibm_macr(const Params<typename Compu
         const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
         const Node<T> nd = make_node(p, idx);
         const uint32_t word = node_word<MASK>(p, nd);
+        const GlobalPC<Q, S, T, IN> pc{p, nd};
         T g[Q];
         if (p.direct || !w_used(word)) {
-            load_direct<Q, S, T>(g, p, nd);
+            load_direct<Q, T>(g, pc);
         } else {
-            gather<Q, S, T>(g, p, nd);
-            if (w_scheme(word) != kNull) apply_bc<Q, S, T>(g, p, nd, word);
+            gather<Q, T>(g, pc);
+            if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
         }
         const Moments<Q, 0, T> m = moments<Q, 0, T>(g, p, (T)0, const_force(p));
         p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
@@ -652,8 +746,10 @@ __global__ __launch_bounds__(kBlock) void k_ibm_macr(const Params<typename Compu
     }
 }
 
-// Equilibrium deviations from (rho, u) (gpu_f_eq, src/CUDA/globalFunctions.h:50-56).
-template <int Q, typename S>
+// Equilibrium deviations from (rho, u) (gpu_f_eq, src/CUDA/globalFunctions.h:50-56),
+// as populations or as coefficients (A = rho-1 - 1.5 rho u.u, B = 3 rho u,
+// G_aa = 4.5 rho u_a^2, G_ab = 9 rho u_a u_b).
+template <int Q, typename S, int OUT = kRepPop>
 __global__ __launch_bounds__(kBlock) void k_fill_feq(S* state, int64_t stride, uint32_t n, const double* rho,
                                                      const double* ux, const double* uy, const double* uz) {
     using T = typename ComputeOf<S>::type;
@@ -661,6 +757,14 @@ __global__ __launch_bounds__(kBlock) void k_fill_feq(S* state, int64_t stride, u
     if (idx >= n) return;
     const T r = (T)rho[idx], vx = (T)ux[idx], vy = (T)uy[idx], vz = (T)uz[idx];
     const T usq = vx * vx + vy * vy + vz * vz;
+    if (OUT == kRepMom) {
+        const T c[kCoef] = {(r - (T)1) - (T)1.5 * r * usq, (T)3 * r * vx, (T)3 * r * vy, (T)3 * r * vz,
+                            (T)4.5 * r * vx * vx, (T)4.5 * r * vy * vy, (T)4.5 * r * vz * vz,
+                            (T)9 * r * vx * vy, (T)9 * r * vx * vz, (T)9 * r * vy * vz};
+#pragma unroll
+        for (int k = 0; k < kCoef; k++) state[(int64_t)k * stride + idx] = (S)c[k];
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         const T cu = (T)kCX[i] * vx + (T)kCY[i] * vy + (T)kCZ[i] * vz;

@@ -1,0 +1,134 @@
+// Time step of the moment-coefficient representation (kRepMom) with LDS
+// staging: one workgroup owns a TX x TY tile of the xy-plane and marches over
+// a chunk of z-planes. The 10 post-collision coefficients of planes z-1, z
+// and z+1 (tile plus a one-node halo in x and y) sit in a three-slot LDS ring;
+// the loads of plane z+2 are issued into registers before plane z is
+// computed. Every node reconstructs its Q pulled populations from LDS, applies
+// its boundary condition (which reads post-collision populations of the node
+// and its neighbours from the same LDS planes), collides and stores its 10
+// coefficients: 10 coefficient reads (plus the halo, served mostly by L2) and
+// 10 writes per node instead of Q and Q.
+#pragma once
+#include "kernels_impl.h"
+
+namespace vlbm {
+
+constexpr int kTileX = 64;
+template <typename T> struct TileShape {
+    static constexpr int TY = sizeof(T) == 4 ? 4 : 2;     // LDS ring of 47.5 KB (FP32) / 63.4 KB (FP64)
+    static constexpr int NT = kTileX * TY;
+    static constexpr int HX = kTileX + 2, HY = TY + 2, HP = HX * HY;
+    static constexpr int PER = (HP + NT - 1) / NT;       // halo positions per thread
+};
+
+// Post-collision population i of the node at offset (dx, dy, dz) from the
+// tile node at LDS position base, planes in slots sm_m (z-1), s0 (z), sp (z+1).
+template <int Q, typename T>
+struct TilePC {
+    const T* sm;
+    int s_m, s_0, s_p;
+    int base;
+    __device__ __forceinline__ T operator()(int i, int dx, int dy, int dz) const {
+        using SH = TileShape<T>;
+        const int slot = dz < 0 ? s_m : (dz > 0 ? s_p : s_0);
+        const T* q = sm + slot * (kCoef * SH::HP) + base + dy * SH::HX + dx;
+        return recon<Q, T, T>(i, q, SH::HP);
+    }
+};
+
+__device__ __forceinline__ int wrap_mod(int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// One step over the lattice, coefficient planes src -> dst (p.src, p.dst).
+// Grid: ceil(nx/TX) * ceil(ny/TY) * ceil(nz/zc) workgroups of TX*TY threads.
+template <int Q, typename T, int RHEO, bool MASK, bool IBM>
+__global__ __launch_bounds__(TileShape<T>::NT) void k_step_tile(const Params<T> p, int zc) {
+    using SH = TileShape<T>;
+    constexpr int TX = kTileX, TY = SH::TY, NT = SH::NT, HX = SH::HX, HP = SH::HP, PER = SH::PER;
+    __shared__ T sm[3 * kCoef * HP];
+    const int ntx = (p.nx + TX - 1) / TX, nty = (p.ny + TY - 1) / TY;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx) * TX, y0 = ((b / ntx) % nty) * TY, z0 = (b / (ntx * nty)) * zc;
+    const int z1 = min(z0 + zc, p.nz);
+    const int t = threadIdx.x, lx = t % TX, ly = t / TX;
+    const uint32_t plane = (uint32_t)p.nx * (uint32_t)p.ny;
+    const T* src = (const T*)p.src;
+
+    // halo positions owned by this thread: in-plane offsets of their nodes
+    uint32_t poff[PER];
+    bool pok[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int r = t + q * NT;
+        pok[q] = r < HP;
+        const int hx = r % HX, hy = r / HX;
+        poff[q] = (uint32_t)wrap_mod(y0 + hy - 1, p.ny) * (uint32_t)p.nx + (uint32_t)wrap_mod(x0 + hx - 1, p.nx);
+    }
+    T reg[PER][kCoef];
+    auto fetch = [&](int zg) {
+        const uint32_t zo = (uint32_t)wrap_mod(zg, p.nz) * plane;
+#pragma unroll
+        for (int q = 0; q < PER; q++)
+            if (pok[q]) {
+#pragma unroll
+                for (int k = 0; k < kCoef; k++) reg[q][k] = src[(int64_t)k * p.stride + zo + poff[q]];
+            }
+    };
+    auto stash = [&](int slot) {
+#pragma unroll
+        for (int q = 0; q < PER; q++)
+            if (pok[q]) {
+#pragma unroll
+                for (int k = 0; k < kCoef; k++) sm[(slot * kCoef + k) * HP + t + q * NT] = reg[q][k];
+            }
+    };
+
+    const int x = x0 + lx, y = y0 + ly;
+    const bool active = x < p.nx && y < p.ny;
+    Node<T> nd;
+    nd.x = x; nd.y = y;
+    nd.xm = x == 0 ? p.nx - 1 : x - 1;
+    nd.xp = x == p.nx - 1 ? 0 : x + 1;
+    nd.ym = y == 0 ? p.ny - 1 : y - 1;
+    nd.yp = y == p.ny - 1 ? 0 : y + 1;
+
+    fetch(z0 - 1); stash(2);
+    fetch(z0); stash(0);
+    fetch(z0 + 1);
+    int s_m = 2, s_0 = 0, s_p = 1;
+    for (int z = z0; z < z1; z++) {
+        stash(s_p);
+        __syncthreads();
+        if (z + 1 < z1) fetch(z + 2);          // in flight while plane z is computed
+        if (active) {
+            nd.z = z;
+            nd.zm = z == 0 ? p.nz - 1 : z - 1;
+            nd.zp = z == p.nz - 1 ? 0 : z + 1;
+            nd.idx = (uint32_t)z * plane + (uint32_t)y * (uint32_t)p.nx + (uint32_t)x;
+            const uint32_t word = node_word<MASK>(p, nd);
+            if (w_used(word)) {
+                const TilePC<Q, T> pc{sm, s_m, s_0, s_p, (ly + 1) * HX + (lx + 1)};
+                T g[Q];
+                gather<Q, T>(g, pc);
+                if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
+                collide_store<Q, T, RHEO, IBM, kRepMom>(p, nd.idx, nd, g);
+            }
+        }
+        __syncthreads();
+        const int old = s_m;
+        s_m = s_0; s_0 = s_p; s_p = old;
+    }
+}
+
+// z-chunk per workgroup: 32 planes, fewer when the lattice needs more
+// workgroups to fill the device.
+inline int tile_zchunk(int nx, int ny, int nz, int ty) {
+    const long tiles = (long)((nx + kTileX - 1) / kTileX) * ((ny + ty - 1) / ty);
+    int zc = 32;
+    while (zc > 4 && tiles * ((nz + zc - 1) / zc) < 2048) zc /= 2;
+    return zc < nz ? zc : nz;
+}
+
+}  // namespace vlbm

@@ -143,14 +143,22 @@ constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
 // the flagged tiles; flagged tiles keep the macroscopics of the IBM stage.
 enum AuxMode : int { kAuxMoments = 0, kAuxMaterialize = 1, kAuxMomentsOutsideTiles = 2 };
 
+// State representations: Q planes of post-collision deviations, or the 10
+// coefficient planes of the regularized post-collision state.
+enum Rep : int { kRepPop = 0, kRepMom = 1 };
+constexpr int kCoef = 10;
+
 // Launchers implemented in kernels.hip. prec: 8, 4 or 2 (storage bytes).
+// in_rep / out_rep: state representation of the source and destination
+// lattices, 0 = populations, 1 = moment coefficients (float / double only;
+// coefficients as output need in_rep 1 or a direct step)
 int launch_step(int q, int prec, int rheo, bool mask_geom, bool direct, const void* params, void* stream,
-                bool ibm = false);
-int launch_aux(int q, int prec, int rheo, bool mask_geom, const void* params, void* stream);
+                bool ibm = false, int in_rep = 0, int out_rep = 0);
+int launch_aux(int q, int prec, int rheo, bool mask_geom, const void* params, void* stream, int in_rep = 0);
 int launch_reduce(int prec, const void* rho, const void* ux, const void* uy, const void* uz, void* ux_old,
                   void* uy_old, void* uz_old, uint32_t n, double* partials, double* out3, void* stream);
 int launch_fill_feq(int q, int prec, void* state, int64_t stride, uint32_t n, const double* rho,
-                    const double* ux, const double* uy, const double* uz, void* stream);
+                    const double* ux, const double* uy, const double* uz, void* stream, int out_rep = 0);
 int launch_convert(int prec, void* buf, double* dbl, size_t count, int to_double, void* stream);
 int launch_copy_probe(const void* src, void* dst, size_t bytes, void* stream);
 

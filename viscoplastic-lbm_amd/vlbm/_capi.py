@@ -16,6 +16,7 @@ VLBM_OK = 0
 ERRORS = {1: "invalid argument", 2: "device error", 3: "allocation failed", 4: "I/O error", 5: "unsupported"}
 
 FP64, FP32, FP16 = 8, 4, 2
+LAYOUT_AUTO, LAYOUT_POPULATIONS, LAYOUT_MOMENTS = 0, 1, 2
 NEWTONIAN, BINGHAM, POWERLAW = 0, 1, 2
 
 
@@ -39,6 +40,7 @@ class VlbmDesc(C.Structure):
         ("zouhe_textbook", C.c_int),
         ("collision", C.c_int),
         ("les_const", C.c_double),
+        ("layout", C.c_int),
     ]
 
 

@@ -23,6 +23,7 @@ file); here they are optional extra keys with the reference defaults:
     zouhe_textbook: false                 # textbook Zou-He instead of the reference lines
     collision: regularized                # or bgk (collisionSchemes/recursiveCollisionForceNoReg.txt)
     les_const: 0.0                        # LES_MODEL Implicit_const, Newtonian only (lbm.cu:226-242)
+    layout: auto                          # state in HBM: auto | populations | moments
     checkpoint_save: 0                    # CHECKPOINT_SAVE interval (var.h:69-71)
     load_checkpoint: null                 # folder to resume from: LOAD_CHECKPOINT with
                                           # SIMULATION_FOLDER_LOAD_CHECKPOINT (var.h:77-84)
@@ -57,6 +58,7 @@ BUILDERS = {
 PRECISIONS = {"double": _capi.FP64, "single": _capi.FP32, "half": _capi.FP16}
 MODELS = {None: _capi.NEWTONIAN, "BINGHAM": _capi.BINGHAM, "POWERLAW": _capi.POWERLAW}
 COLLISIONS = {"regularized": 0, "bgk": 1}
+LAYOUTS = {"auto": 0, "populations": 1, "moments": 2}
 
 
 @dataclass
@@ -107,6 +109,7 @@ class SimConfig:
     zouhe_textbook: bool = False
     collision: str = "regularized"
     les_const: float = 0.0
+    layout: str = "auto"
     checkpoint_save: int = 0
     load_checkpoint: Optional[str] = None
     # immersed boundary (use_ibm): vlbm_ibm_desc fields (ibmVar.h /
@@ -174,6 +177,7 @@ class SimConfig:
         d.zouhe_textbook = 1 if self.zouhe_textbook else 0
         d.collision = COLLISIONS[self.collision]
         d.les_const = self.les_const
+        d.layout = LAYOUTS[self.layout]
         return d
 
     def replace(self, **kw) -> "SimConfig":
@@ -214,6 +218,7 @@ def _validate(c: SimConfig) -> None:
         need(isinstance(getattr(c, name), bool), f"{name} must be a boolean")
     need(c.checkpoint_save >= 0, "checkpoint_save must be >= 0")
     need(c.collision in COLLISIONS, "collision must be 'regularized' or 'bgk'")
+    need(c.layout in LAYOUTS, "layout must be 'auto', 'populations' or 'moments'")
     need(c.les_const >= 0, "les_const must be >= 0")
     need(c.les_const == 0 or c.non_newtonian_model is None, "LES applies to Newtonian fluids only")
     need(c.load_checkpoint is None or isinstance(c.load_checkpoint, str), "load_checkpoint must be a folder name")

@@ -40,7 +40,7 @@ class VlbmBatchDesc(C.Structure):
     _fields_ = [
         ("hbm_budget", C.c_size_t), ("max_jobs", C.c_int), ("q", C.c_int), ("precision", C.c_int),
         ("check_every", C.c_int), ("steps_per_launch", C.c_int), ("block", C.c_int), ("grid", C.c_int),
-        ("use_graph", C.c_int), ("tune_cache", C.c_char_p),
+        ("use_graph", C.c_int), ("tune_cache", C.c_char_p), ("max_resident_nodes", C.c_size_t),
     ]
 
 
@@ -359,17 +359,20 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--tune-cache", default=None)
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--resident-nodes", type=float, default=float(1 << 24),
+                    help="admit while the resident lattices hold at most this many nodes (0: no limit)")
     a = ap.parse_args()
     jobs = make_jobs(a.jobs, a.rate, a.seed)
     progress = lambda s: print(s, flush=True)   # noqa: E731
     t = time.perf_counter()
     res = sweep(jobs, a.resid, a.max_steps, a.check_every, realtime=True,
                 hbm_budget=int(a.budget_gb * (1 << 30)), max_jobs=256, tune_cache=a.tune_cache,
-                log=lambda s: None)
+                max_resident_nodes=int(a.resident_nodes), log=lambda s: None)
     out = {"metric": "serving sweep: jobs/s, aggregate MLUPS, p50/p99 time from arrival to result",
            "batched": res.summary(), "batch_config": res.config,
            "workload": {"jobs": a.jobs, "rate_per_s": a.rate, "seed": a.seed, "max_steps": a.max_steps,
                         "resid_max": a.resid, "check_every": a.check_every,
+                        "max_resident_nodes": int(a.resident_nodes),
                         "lattice": "Nx U[32,256], Ny U[32,128], Nz U[16,128]; Bn U[0,5], tau U[0.55,1.5], "
                                    "Fx U[1e-6,1e-5]; D3Q19 FP32 Bingham channel"}}
     progress(f"batched sweep done in {time.perf_counter() - t:.1f} s")
