@@ -601,6 +601,7 @@ __device__ __forceinline__ bool node_stencil(const IbmParams& P, const double po
 // the others read theirs with shuffles.
 constexpr int kInterpBlock = 256;
 constexpr int kSphereBlock = 1024;
+constexpr int kRowLanesShare = 16;   // lanes per node in the row-parallel kernels
 
 struct PointOf {
     int xi, yj, zk;
@@ -623,6 +624,33 @@ __device__ __forceinline__ PointOf stencil_point(const Stencil& st, const double
             pt.zk >= st.mn[2] && pt.zk <= st.mx[2];
     pt.a = pt.in ? (wz * wy) * wx : 0.0;
     return pt;
+}
+
+// 16-lane groups (one node per group, lane r = one (z, y) row): lanes 0..11
+// of the group evaluate the node's 12 one-dimensional kernel weights once and
+// every lane reads its row weight phi_z phi_y and the four x weights with
+// shuffles inside the group. Called by all lanes (uniform control flow).
+struct RowWeights {
+    double zy;         // phi_z(zk) phi_y(yj) of the lane's row
+    double x[4];       // phi_x of the four x points
+};
+
+__device__ __forceinline__ RowWeights row_weights(int bx, int by, int bz, double px, double py, double pz, bool ok,
+                                                  int r) {
+    double wl = 0.0;
+    if (ok && r < 12) {
+        const int a = r >> 2, j = r & 3;
+        // the same expression as peskin4(st.base[a] + j - pos[a])
+        const int b = a == 0 ? bx : (a == 1 ? by : bz);
+        const double pa = a == 0 ? px : (a == 1 ? py : pz);
+        wl = peskin4(b + j - pa);
+    }
+    const int g = (int)(threadIdx.x & 63) & ~(kRowLanesShare - 1);
+    RowWeights w;
+    w.zy = __shfl(wl, g + 8 + (r >> 2), 64) * __shfl(wl, g + 4 + (r & 3), 64);
+#pragma unroll
+    for (int xi = 0; xi < 4; xi++) w.x[xi] = __shfl(wl, g + xi, 64);
+    return w;
 }
 
 // (1) one workgroup per chunk of up to 64 nodes of one sphere, 16 lanes per
@@ -655,8 +683,11 @@ __global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFlu
         if (act) {
             pos[0] = P.px[i]; pos[1] = P.py[i]; pos[2] = P.pz[i];
             st = make_stencil(P, pos);
+        }
+        const RowWeights rw = row_weights(st.base[0], st.base[1], st.base[2], pos[0], pos[1], pos[2], act && st.valid, r);
+        if (act) {
             if (st.valid && zk >= st.mn[2] && zk <= st.mx[2] && yj >= st.mn[1] && yj <= st.mx[1]) {
-                const double a1 = peskin4(st.base[2] + zk - pos[2]) * peskin4(st.base[1] + yj - pos[1]);
+                const double a1 = rw.zy;
                 const size_t row = ((size_t)stencil_coord(P, st, 2, zk) * P.ny + stencil_coord(P, st, 1, yj)) * P.nx;
                 // all four x points at once (weight 0 and an in-range node for
                 // points the wall clips), so the loads issue together
@@ -665,7 +696,7 @@ __global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFlu
 #pragma unroll
                 for (int xi = 0; xi < 4; xi++) {
                     const bool in = xi >= st.mn[0] && xi <= st.mx[0];
-                    wx[xi] = in ? a1 * peskin4(st.base[0] + xi - pos[0]) : 0.0;
+                    wx[xi] = in ? a1 * rw.x[xi] : 0.0;
                     kx[xi] = row + stencil_coord(P, st, 0, in ? xi : st.mn[0]);
                 }
                 T tr[4], tx[4], ty[4], tz[4], fx[4], fy[4], fz[4];
@@ -775,12 +806,20 @@ __global__ __launch_bounds__(kSphereBlock) void k_ibm_spread_box(IbmParams P, Ib
     const int r = threadIdx.x & (kRowLanes - 1);
     const int zk = r >> 2, yj = r & 3;
     const int items = c.n_nodes * kRowLanes;
-    for (int k = threadIdx.x; k < items; k += kSphereBlock) {
-        const int i = c.first_node + k / kRowLanes;
-        const double pos[3] = {P.px[i], P.py[i], P.pz[i]};
-        const Stencil st = make_stencil(P, pos);
-        if (!st.valid || zk < st.mn[2] || zk > st.mx[2] || yj < st.mn[1] || yj > st.mx[1]) continue;
-        const double a1 = peskin4(st.base[2] + zk - pos[2]) * peskin4(st.base[1] + yj - pos[1]);
+    for (int b0 = 0; b0 < items; b0 += kSphereBlock) {      // uniform trip count for the shuffles
+        const int k = b0 + (int)threadIdx.x;
+        const bool act = k < items;
+        const int i = c.first_node + (act ? k / kRowLanes : 0);
+        double pos[3] = {0, 0, 0};
+        Stencil st;
+        st.valid = false;
+        if (act) {
+            pos[0] = P.px[i]; pos[1] = P.py[i]; pos[2] = P.pz[i];
+            st = make_stencil(P, pos);
+        }
+        const RowWeights rw = row_weights(st.base[0], st.base[1], st.base[2], pos[0], pos[1], pos[2], act && st.valid, r);
+        if (!act || !st.valid || zk < st.mn[2] || zk > st.mx[2] || yj < st.mn[1] || yj > st.mx[1]) continue;
+        const double a1 = rw.zy;
         const int cz = stencil_coord(P, st, 2, zk), cy = stencil_coord(P, st, 1, yj);
         int bz = st.base[2] + zk - o[2], by = st.base[1] + yj - o[1];
         if (P.periodic[2]) bz = ((bz % P.nz) + P.nz) % P.nz;
@@ -789,7 +828,7 @@ __global__ __launch_bounds__(kSphereBlock) void k_ibm_spread_box(IbmParams P, Ib
 #pragma unroll
         for (int xi = 0; xi < 4; xi++) {
             if (xi < st.mn[0] || xi > st.mx[0]) continue;
-            const double a = a1 * peskin4(st.base[0] + xi - pos[0]);
+            const double a = a1 * rw.x[xi];
             const float vx = (float)(-dF[0] * a), vy = (float)(-dF[1] * a), vz = (float)(-dF[2] * a);
             int bx = st.base[0] + xi - o[0];
             if (P.periodic[0]) bx = ((bx % P.nx) + P.nx) % P.nx;
