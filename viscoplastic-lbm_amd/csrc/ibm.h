@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstdint>
 
+#include "lattice.h"
+
 namespace vlbm {
 
 constexpr int kMaxColl = 13;        // history slots: 7 walls + 6 partners (MAX_ACTIVE_COLLISIONS)
@@ -92,9 +94,8 @@ constexpr int kChunkNodes = 64;
 // updated while forces spread.
 template <typename T>
 struct IbmFluid {
-    T *rho, *ux, *uy, *uz;     // rho, u of the post-step populations with the constant force
+    StageRec<T>* stage;        // rho and u + (f - f0) / 2 rho of the listed nodes
     T *fx, *fy, *fz;           // force field (read by the next step in listed tiles)
-    T f0x, f0y, f0z;           // constant force
     T *out_rho, *out_ux, *out_uy, *out_uz;   // saved macroscopics of the handle
 };
 

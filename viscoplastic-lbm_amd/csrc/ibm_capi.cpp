@@ -34,7 +34,7 @@ void update_cells(const vlbm_ctx* h, IbmState& s) {
     s.P.box_edge = 0;
     if (h->tbytes == 4 && s.r_max > 0) {
         const int edge = ibm_box_edge(s.r_max);
-        const size_t bytes = 3 * sizeof(float) * (size_t)edge * edge * edge;
+        const size_t bytes = 3 * sizeof(int) * (size_t)edge * edge * edge;
         hipDeviceProp_t prop;
         int dev = 0;
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
@@ -138,9 +138,8 @@ template <typename T>
 IbmFluid<T> fluid_of(const vlbm_ctx* h) {
     const IbmState& s = *h->ibm;
     IbmFluid<T> fl;
-    fl.rho = (T*)s.macr[0]; fl.ux = (T*)s.macr[1]; fl.uy = (T*)s.macr[2]; fl.uz = (T*)s.macr[3];
+    fl.stage = (StageRec<T>*)s.stage;
     fl.fx = (T*)s.force[0]; fl.fy = (T*)s.force[1]; fl.fz = (T*)s.force[2];
-    fl.f0x = (T)h->d.fx; fl.f0y = (T)h->d.fy; fl.f0z = (T)h->d.fz;
     fl.out_rho = (T*)h->macr[0]; fl.out_ux = (T*)h->macr[1]; fl.out_uy = (T*)h->macr[2]; fl.out_uz = (T*)h->macr[3];
     return fl;
 }
@@ -280,7 +279,7 @@ int ibm_stage(vlbm_ctx* h) {
         p.src = state_src(h);
         p.dst = h->lat[1 - h->cur];
         p.direct = h->direct ? 1 : 0;
-        p.m_rho = (T*)s.macr[0]; p.m_ux = (T*)s.macr[1]; p.m_uy = (T*)s.macr[2]; p.m_uz = (T*)s.macr[3];
+        p.ibm_stage = (StageRec<T>*)s.stage;
         return check_launch(launch_ibm_macr(d.q, h->sbytes, h->mask_geom, pv, nullptr, s.P.tile_list, s.P.tile_count,
                                             max_tiles, h->stream, state_rep(h)),
                             "IBM macroscopics kernel");
@@ -394,7 +393,7 @@ int vlbm_ibm_enable(vlbm_h h, const vlbm_ibm_desc* d) {
     A((void**)&P.chunk_sums, mc * 6 * sizeof(double));
     A((void**)&P.error, 4);
     for (int a = 0; a < 3; a++) A(&s.force[a], h->n * h->tbytes);
-    for (int a = 0; a < 4; a++) A(&s.macr[a], h->n * h->tbytes);
+    A(&s.stage, h->n * 4 * h->tbytes);
     if (rc) { ibm_release(h); return rc; }
     hipError_t e = hipSuccess;
     for (int a = 0; a < 3 && e == hipSuccess; a++) e = hipMemsetAsync(s.force[a], 0, h->n * h->tbytes, h->stream);

@@ -86,13 +86,19 @@ __device__ __forceinline__ Stencil make_stencil(const IbmParams& P, const double
     return s;
 }
 
+// v mod n in [0, n). Stencil and box coordinates lie within one period of
+// the range (valid stencils start in [-4, n + 3]), so one add or subtract
+// wraps them; values farther out (axes shorter than the stencil's reach)
+// take the integer modulo.
+__device__ __forceinline__ int wrap_axis(int v, int n) {
+    v = v < 0 ? v + n : (v >= n ? v - n : v);
+    if ((unsigned)v >= (unsigned)n) v = ((v % n) + n) % n;
+    return v;
+}
+
 __device__ __forceinline__ int stencil_coord(const IbmParams& P, const Stencil& s, int a, int j) {
-    int c = s.base[a] + j;
-    if (P.periodic[a]) {
-        const int n = axis_n(P, a);
-        c = ((c % n) + n) % n;
-    }
-    return c;
+    const int c = s.base[a] + j;
+    return P.periodic[a] ? wrap_axis(c, axis_n(P, a)) : c;
 }
 
 // ---------------------------------------------------------------------------
@@ -654,9 +660,10 @@ __device__ __forceinline__ RowWeights row_weights(int bx, int by, int bz, double
 }
 
 // (1) one workgroup per chunk of up to 64 nodes of one sphere, 16 lanes per
-// node (one per (z, y) row of the stencil): interpolated rho and u
-// (u + (f - f0) / 2 rho, the velocity after the iterations so far) over the
-// lane's row, the 16 row sums combined by a butterfly; the node's force
+// node (one per (z, y) row of the stencil): interpolated rho and u (the stage
+// records' u + (f - f0) / 2 rho, the velocity after the iterations so far,
+// one vector load per point) over the lane's row, the 16 row sums combined by
+// a butterfly; the node's force
 // dF = 2 rho dA thickness (u - u_node) and its moment about the centre; the
 // chunk's force and moment sums in a fixed order (strided per-thread partial
 // sums, wave64 tree), added to the sphere by k_ibm_add_chunks.
@@ -672,18 +679,24 @@ __global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFlu
     const int zk = r >> 2, yj = r & 3;
     const int items = P.chunk_count[chunk] * kRowLanes;
     double acc[6] = {0, 0, 0, 0, 0, 0};
+    // position of the node of item k (the chunk's first node past the end);
+    // the next item's is loaded while the current one is interpolated
+    auto fetch = [&](int k, double (&q)[3]) {
+        const int i = first + (k < items ? k / kRowLanes : 0);
+        q[0] = P.px[i]; q[1] = P.py[i]; q[2] = P.pz[i];
+    };
+    double nxt[3] = {0, 0, 0};
+    if (items > 0) fetch((int)threadIdx.x, nxt);
     for (int b0 = 0; b0 < items; b0 += kInterpBlock) {      // uniform trip count for the shuffles
         const int k = b0 + (int)threadIdx.x;
         const bool act = k < items;
         const int i = first + k / kRowLanes;
         double pv[4] = {0, 0, 0, 0};
-        double pos[3] = {0, 0, 0};
+        const double pos[3] = {nxt[0], nxt[1], nxt[2]};
+        if (b0 + kInterpBlock < items) fetch(k + kInterpBlock, nxt);
         Stencil st;
         st.valid = false;
-        if (act) {
-            pos[0] = P.px[i]; pos[1] = P.py[i]; pos[2] = P.pz[i];
-            st = make_stencil(P, pos);
-        }
+        if (act) st = make_stencil(P, pos);
         const RowWeights rw = row_weights(st.base[0], st.base[1], st.base[2], pos[0], pos[1], pos[2], act && st.valid, r);
         if (act) {
             if (st.valid && zk >= st.mn[2] && zk <= st.mx[2] && yj >= st.mn[1] && yj <= st.mx[1]) {
@@ -699,19 +712,15 @@ __global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFlu
                     wx[xi] = in ? a1 * rw.x[xi] : 0.0;
                     kx[xi] = row + stencil_coord(P, st, 0, in ? xi : st.mn[0]);
                 }
-                T tr[4], tx[4], ty[4], tz[4], fx[4], fy[4], fz[4];
+                StageRec<T> v[4];
+#pragma unroll
+                for (int xi = 0; xi < 4; xi++) v[xi] = fl.stage[kx[xi]];
 #pragma unroll
                 for (int xi = 0; xi < 4; xi++) {
-                    tr[xi] = fl.rho[kx[xi]]; tx[xi] = fl.ux[kx[xi]]; ty[xi] = fl.uy[kx[xi]]; tz[xi] = fl.uz[kx[xi]];
-                    fx[xi] = fl.fx[kx[xi]]; fy[xi] = fl.fy[kx[xi]]; fz[xi] = fl.fz[kx[xi]];
-                }
-#pragma unroll
-                for (int xi = 0; xi < 4; xi++) {
-                    const T h = (T)0.5 / tr[xi];
-                    pv[0] += (double)tr[xi] * wx[xi];
-                    pv[1] += (double)(tx[xi] + (fx[xi] - fl.f0x) * h) * wx[xi];
-                    pv[2] += (double)(ty[xi] + (fy[xi] - fl.f0y) * h) * wx[xi];
-                    pv[3] += (double)(tz[xi] + (fz[xi] - fl.f0z) * h) * wx[xi];
+                    pv[0] += (double)v[xi].rho * wx[xi];
+                    pv[1] += (double)v[xi].ux * wx[xi];
+                    pv[2] += (double)v[xi].uy * wx[xi];
+                    pv[3] += (double)v[xi].uz * wx[xi];
                 }
             }
         }
@@ -769,6 +778,26 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_add_chunks(IbmParams P) {
     for (int a = 0; a < 3; a++) { c.f[a] += t[a]; c.M[a] += t[3 + a]; }
 }
 
+// Adds the spread force v to node kk of density rho: to the force field and,
+// as v / 2 rho, to the node's stage velocity (velAux of spreadForce,
+// ibm.cu:489-509).
+template <typename T>
+__device__ __forceinline__ void add_spread(const IbmFluid<T>& fl, size_t kk, T rho, T vx, T vy, T vz) {
+    unsafeAtomicAdd(fl.fx + kk, vx);
+    unsafeAtomicAdd(fl.fy + kk, vy);
+    unsafeAtomicAdd(fl.fz + kk, vz);
+    StageRec<T>* r = fl.stage + kk;
+    const T h = (T)0.5 / rho;
+    unsafeAtomicAdd(&r->ux, vx * h);
+    unsafeAtomicAdd(&r->uy, vy * h);
+    unsafeAtomicAdd(&r->uz, vz * h);
+}
+
+template <typename T>
+__device__ __forceinline__ void add_spread(const IbmFluid<T>& fl, size_t kk, T vx, T vy, T vz) {
+    add_spread<T>(fl, kk, fl.stage[kk].rho, vx, vy, vz);
+}
+
 // (2a) spreading into the force field with global atomics, one wave per node
 // (FP64 fluids, and FP32 when the LDS box does not fit).
 template <typename T>
@@ -783,89 +812,143 @@ __global__ __launch_bounds__(kInterpBlock) void k_ibm_spread(IbmParams P, IbmFlu
     if (!pt.in) return;
     const size_t kk = ((size_t)stencil_coord(P, st, 2, pt.zk) * P.ny + stencil_coord(P, st, 1, pt.yj)) * P.nx +
                       stencil_coord(P, st, 0, pt.xi);
-    unsafeAtomicAdd(fl.fx + kk, (T)(-P.pdx[i] * pt.a));
-    unsafeAtomicAdd(fl.fy + kk, (T)(-P.pdy[i] * pt.a));
-    unsafeAtomicAdd(fl.fz + kk, (T)(-P.pdz[i] * pt.a));
+    add_spread<T>(fl, kk, (T)(-P.pdx[i] * pt.a), (T)(-P.pdy[i] * pt.a), (T)(-P.pdz[i] * pt.a));
 }
 
 // (2b) FP32 fluids: one workgroup per sphere, 16 lanes per node (one stencil
 // row each), collects the spread force in an LDS box of edge^3 nodes around
-// the sphere (LDS float atomics) and adds the box to the force field with one
-// global atomic per box node and component. Points outside the box (nodes
-// that left the sphere's radius) go straight to global memory.
+// the sphere and adds the box to the force field and the stage velocities
+// with one global atomic per box node, field and component. Points outside
+// the box (nodes that left the sphere's radius) go straight to global memory.
+//
+// The box holds fixed-point integers (LDS integer atomics; float LDS atomics
+// cost several times more): component a of a contribution -dF_a w is stored
+// as rint(-dF_a w s_a) with s_a = 2^30 / (sum over the sphere's nodes of
+// |dF_a| / 8). A box node takes at most one contribution per node, each with
+// weight w <= 1/8 (the 4-point kernel is at most 1/2 per axis), so every box
+// sum stays within 2^30 plus the rounding, inside int32. The sums are exact
+// and independent of the order of the atomics; the rounding of a
+// contribution is at most 2^-31 of the sphere's force sum. A sphere whose
+// force sum is not finite spreads through the global float atomics, so that
+// the fluid sees the same NaN or infinity as the sphere.
+__device__ __forceinline__ int box_fixed(double v, double s) { return (int)rint(v * s); }
+
 __global__ __launch_bounds__(kSphereBlock) void k_ibm_spread_box(IbmParams P, IbmFluid<float> fl) {
-    extern __shared__ float box[];                 // [3][edge^3]
+    extern __shared__ int box[];                   // [3][edge^3]
+    __shared__ double red[3][kSphereBlock / 64];
+    __shared__ double scale[3];
+    __shared__ int finite[3];
     const IbmSphere& c = P.sph[blockIdx.x];
     const int edge = P.box_edge;
     const int e3 = edge * edge * edge;
-    for (int e = threadIdx.x; e < 3 * e3; e += kSphereBlock) box[e] = 0.f;
+    for (int e = threadIdx.x; e < 3 * e3; e += kSphereBlock) box[e] = 0;
     const int half = edge / 2 - 3;                 // ceil(radius)
     int o[3];
     for (int a = 0; a < 3; a++) o[a] = (int)floor(c.pos[a]) - half - 2;
-    __syncthreads();
+    const int first = c.first_node;
+    const int items = c.n_nodes * kRowLanes;
+    {
+        double sa[3] = {0, 0, 0};
+        for (int k = threadIdx.x; k < c.n_nodes; k += kSphereBlock) {
+            sa[0] += fabs(P.pdx[first + k]); sa[1] += fabs(P.pdy[first + k]); sa[2] += fabs(P.pdz[first + k]);
+        }
+        for (int off = 32; off > 0; off >>= 1)
+            for (int a = 0; a < 3; a++) sa[a] += __shfl_down(sa[a], off, 64);
+        if ((threadIdx.x & 63) == 0)
+            for (int a = 0; a < 3; a++) red[a][threadIdx.x >> 6] = sa[a];
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            double t = 0;
+            for (int w = 0; w < kSphereBlock / 64; w++) t += red[threadIdx.x][w];
+            scale[threadIdx.x] = t > 0 ? 1073741824.0 / (0.125 * t) : 1.0;
+            finite[threadIdx.x] = isfinite(t) ? 1 : 0;
+        }
+    }
     const int r = threadIdx.x & (kRowLanes - 1);
     const int zk = r >> 2, yj = r & 3;
-    const int items = c.n_nodes * kRowLanes;
+    // position and force of the node of item k (the sphere's first node past
+    // the end); the next item's are loaded while the current one is spread
+    auto fetch = [&](int k, double (&q)[6]) {
+        const int i = first + (k < items ? k / kRowLanes : 0);
+        q[0] = P.px[i]; q[1] = P.py[i]; q[2] = P.pz[i];
+        q[3] = P.pdx[i]; q[4] = P.pdy[i]; q[5] = P.pdz[i];
+    };
+    double nxt[6] = {0, 0, 0, 0, 0, 0};
+    if (items > 0) fetch((int)threadIdx.x, nxt);
+    __syncthreads();
+    const double sx = scale[0], sy = scale[1], sz = scale[2];
+    const bool use_box = finite[0] && finite[1] && finite[2];
     for (int b0 = 0; b0 < items; b0 += kSphereBlock) {      // uniform trip count for the shuffles
         const int k = b0 + (int)threadIdx.x;
         const bool act = k < items;
-        const int i = c.first_node + (act ? k / kRowLanes : 0);
-        double pos[3] = {0, 0, 0};
+        double q[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) q[j] = nxt[j];
+        if (b0 + kSphereBlock < items) fetch(k + kSphereBlock, nxt);
+        const double pos[3] = {q[0], q[1], q[2]};
         Stencil st;
         st.valid = false;
-        if (act) {
-            pos[0] = P.px[i]; pos[1] = P.py[i]; pos[2] = P.pz[i];
-            st = make_stencil(P, pos);
-        }
+        if (act) st = make_stencil(P, pos);
         const RowWeights rw = row_weights(st.base[0], st.base[1], st.base[2], pos[0], pos[1], pos[2], act && st.valid, r);
         if (!act || !st.valid || zk < st.mn[2] || zk > st.mx[2] || yj < st.mn[1] || yj > st.mx[1]) continue;
         const double a1 = rw.zy;
         const int cz = stencil_coord(P, st, 2, zk), cy = stencil_coord(P, st, 1, yj);
         int bz = st.base[2] + zk - o[2], by = st.base[1] + yj - o[1];
-        if (P.periodic[2]) bz = ((bz % P.nz) + P.nz) % P.nz;
-        if (P.periodic[1]) by = ((by % P.ny) + P.ny) % P.ny;
-        const double dF[3] = {P.pdx[i], P.pdy[i], P.pdz[i]};
+        if (P.periodic[2]) bz = wrap_axis(bz, P.nz);
+        if (P.periodic[1]) by = wrap_axis(by, P.ny);
 #pragma unroll
         for (int xi = 0; xi < 4; xi++) {
             if (xi < st.mn[0] || xi > st.mx[0]) continue;
             const double a = a1 * rw.x[xi];
-            const float vx = (float)(-dF[0] * a), vy = (float)(-dF[1] * a), vz = (float)(-dF[2] * a);
+            const float vx = (float)(-q[3] * a), vy = (float)(-q[4] * a), vz = (float)(-q[5] * a);
             int bx = st.base[0] + xi - o[0];
-            if (P.periodic[0]) bx = ((bx % P.nx) + P.nx) % P.nx;
-            if (bx >= 0 && bx < edge && by >= 0 && by < edge && bz >= 0 && bz < edge) {
+            if (P.periodic[0]) bx = wrap_axis(bx, P.nx);
+            if (use_box && bx >= 0 && bx < edge && by >= 0 && by < edge && bz >= 0 && bz < edge) {
                 const int e = (bz * edge + by) * edge + bx;
-                atomicAdd(box + e, vx);
-                atomicAdd(box + e3 + e, vy);
-                atomicAdd(box + 2 * e3 + e, vz);
+                atomicAdd(box + e, box_fixed(-q[3] * a, sx));
+                atomicAdd(box + e3 + e, box_fixed(-q[4] * a, sy));
+                atomicAdd(box + 2 * e3 + e, box_fixed(-q[5] * a, sz));
             } else {
                 const size_t kk = ((size_t)cz * P.ny + cy) * P.nx + stencil_coord(P, st, 0, xi);
-                unsafeAtomicAdd(fl.fx + kk, vx);
-                unsafeAtomicAdd(fl.fy + kk, vy);
-                unsafeAtomicAdd(fl.fz + kk, vz);
+                add_spread<float>(fl, kk, vx, vy, vz);
             }
         }
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < e3; e += kSphereBlock) {
-        const float vx = box[e], vy = box[e3 + e], vz = box[2 * e3 + e];
-        if (vx == 0.f && vy == 0.f && vz == 0.f) continue;
-        int q[3] = {o[0] + e % edge, o[1] + (e / edge) % edge, o[2] + e / (edge * edge)};
-        bool inside = true;
-        for (int a = 0; a < 3; a++) {
-            const int n = axis_n(P, a);
-            if (P.periodic[a]) q[a] = ((q[a] % n) + n) % n;
-            else if (q[a] < 0 || q[a] >= n) inside = false;
+    // flush in rounds of kFlushBatch box nodes per thread: the densities of a
+    // round are loaded before its atomics are issued, so each load waits for
+    // the previous round's atomics only (memory operations retire in order)
+    constexpr int kFlushBatch = 8;
+    for (int e0 = 0; e0 < e3; e0 += kFlushBatch * kSphereBlock) {
+        size_t kk[kFlushBatch];
+        float rho[kFlushBatch];
+        bool on[kFlushBatch];
+#pragma unroll
+        for (int j = 0; j < kFlushBatch; j++) {
+            const int e = e0 + j * kSphereBlock + (int)threadIdx.x;
+            on[j] = e < e3 && (box[e] != 0 || box[e3 + e] != 0 || box[2 * e3 + e] != 0);
+            int q[3] = {o[0] + e % edge, o[1] + (e / edge) % edge, o[2] + e / (edge * edge)};
+            for (int a = 0; a < 3; a++) {
+                const int n = axis_n(P, a);
+                if (P.periodic[a]) q[a] = wrap_axis(q[a], n);
+                else if (q[a] < 0 || q[a] >= n) on[j] = false;
+            }
+            kk[j] = on[j] ? ((size_t)q[2] * P.ny + q[1]) * P.nx + q[0] : 0;
+            rho[j] = on[j] ? fl.stage[kk[j]].rho : 1.f;
         }
-        if (!inside) continue;
-        const size_t kk = ((size_t)q[2] * P.ny + q[1]) * P.nx + q[0];
-        unsafeAtomicAdd(fl.fx + kk, vx);
-        unsafeAtomicAdd(fl.fy + kk, vy);
-        unsafeAtomicAdd(fl.fz + kk, vz);
+#pragma unroll
+        for (int j = 0; j < kFlushBatch; j++) {
+            const int e = e0 + j * kSphereBlock + (int)threadIdx.x;
+            if (on[j])
+                add_spread<float>(fl, kk[j], rho[j], (float)((double)box[e] / sx), (float)((double)box[e3 + e] / sy),
+                                  (float)((double)box[2 * e3 + e] / sz));
+        }
     }
 }
 
 // saved macroscopics on the listed tiles: rho and the velocity after the
-// stage, u + (f - f0) / 2 rho (the reference's u after its Euler sums)
+// stage, u + (f - f0) / 2 rho (the reference's u after its Euler sums), from
+// the stage records
 template <typename T>
 __global__ __launch_bounds__(kIbmBlock) void k_ibm_output(IbmParams P, IbmFluid<T> fl) {
     const uint32_t total = *P.tile_count * (uint32_t)kTileNodes;
@@ -880,12 +963,11 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_output(IbmParams P, IbmFluid<
         const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
         if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
         const size_t k = ((size_t)z * P.ny + y) * P.nx + x;
-        const T r = fl.rho[k];
-        const T h = (T)0.5 / r;
-        fl.out_rho[k] = r;
-        fl.out_ux[k] = fl.ux[k] + (fl.fx[k] - fl.f0x) * h;
-        fl.out_uy[k] = fl.uy[k] + (fl.fy[k] - fl.f0y) * h;
-        fl.out_uz[k] = fl.uz[k] + (fl.fz[k] - fl.f0z) * h;
+        const StageRec<T> r = fl.stage[k];
+        fl.out_rho[k] = r.rho;
+        fl.out_ux[k] = r.ux;
+        fl.out_uy[k] = r.uy;
+        fl.out_uz[k] = r.uz;
     }
 }
 

@@ -13,7 +13,7 @@ struct IbmState {
     IbmParams P;
     double r_max = 0;
     void* force[3] = {nullptr, nullptr, nullptr};
-    void* macr[4] = {nullptr, nullptr, nullptr, nullptr};   // rho, u of the stage (constant force)
+    void* stage = nullptr;     // StageRec per node: rho, u + (f - f0) / 2 rho of the stage
     std::vector<void*> allocs;
 };
 

@@ -24,7 +24,7 @@ int spread<float>(const IbmParams& P, const IbmFluid<float>& fl, hipStream_t st)
                            st, P, fl);
         return 0;
     }
-    const size_t lds = 3 * sizeof(float) * (size_t)P.box_edge * P.box_edge * P.box_edge;
+    const size_t lds = 3 * sizeof(int) * (size_t)P.box_edge * P.box_edge * P.box_edge;
     static size_t lds_set = 0;
     if (lds > lds_set) {
         if (hipFuncSetAttribute((const void*)k_ibm_spread_box, hipFuncAttributeMaxDynamicSharedMemorySize,

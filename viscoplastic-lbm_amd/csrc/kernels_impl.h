@@ -708,9 +708,10 @@ __global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<
 }
 
 // IBM macroscopics on the flagged tiles (gpuUpdateMacrIBM, src/CUDA/IBM/ibm.cu:544-648):
-// rho and u of the post-step populations with the constant force; the force
-// field of those nodes restarts from the constant force. Grid-stride over the
-// tile list (its length is read on the device).
+// rho and u of the post-step populations with the constant force, as the
+// node's stage record; the force field of those nodes restarts from the
+// constant force. Grid-stride over the tile list (its length is read on the
+// device).
 template <int Q, typename S, bool MASK, int IN = kRepPop>
 __global__ __launch_bounds__(kBlock) void k_ibm_macr(const Params<typename ComputeOf<S>::type> p,
                                                      const uint32_t* tile_list, const uint32_t* tile_count) {
@@ -741,7 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_ibm_macr(const Params<typename Compu
             if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
         }
         const Moments<Q, 0, T> m = moments<Q, 0, T>(g, p, (T)0, const_force(p));
-        p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
+        p.ibm_stage[idx] = StageRec<T>{m.rho, m.ux, m.uy, m.uz};
         ffx[idx] = p.fx; ffy[idx] = p.fy; ffz[idx] = p.fz;
     }
 }

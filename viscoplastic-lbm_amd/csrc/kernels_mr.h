@@ -22,15 +22,18 @@ template <typename T> struct TileShape {
 };
 
 // Post-collision population i of the node at offset (dx, dy, dz) from the
-// tile node at LDS position base, planes in slots sm_m (z-1), s0 (z), sp (z+1).
+// tile node at LDS position base. The ring slots of planes z-1, z and z+1 are
+// packed 4 bits each into one int (slot of plane z+dz at bit 4 (dz+1)): a
+// select among three struct fields becomes an indexed access to the struct
+// in private memory, a shift does not.
 template <int Q, typename T>
 struct TilePC {
     const T* sm;
-    int s_m, s_0, s_p;
+    int slots;
     int base;
     __device__ __forceinline__ T operator()(int i, int dx, int dy, int dz) const {
         using SH = TileShape<T>;
-        const int slot = dz < 0 ? s_m : (dz > 0 ? s_p : s_0);
+        const int slot = (slots >> (4 * (dz + 1))) & 15;
         const T* q = sm + slot * (kCoef * SH::HP) + base + dy * SH::HX + dx;
         return recon<Q, T, T>(i, q, SH::HP);
     }
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(TileShape<T>::NT) void k_step_tile(const Params<T> 
             nd.idx = (uint32_t)z * plane + (uint32_t)y * (uint32_t)p.nx + (uint32_t)x;
             const uint32_t word = node_word<MASK>(p, nd);
             if (w_used(word)) {
-                const TilePC<Q, T> pc{sm, s_m, s_0, s_p, (ly + 1) * HX + (lx + 1)};
+                const TilePC<Q, T> pc{sm, s_m | (s_0 << 4) | (s_p << 8), (ly + 1) * HX + (lx + 1)};
                 T g[Q];
                 gather<Q, T>(g, pc);
                 if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);

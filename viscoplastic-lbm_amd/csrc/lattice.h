@@ -100,6 +100,15 @@ inline FastDiv make_fastdiv(uint32_t d) {
     return f;
 }
 
+// Record of one node in the immersed boundary stage array: rho and the
+// velocity the interpolation reads, u + (f - f0) / 2 rho (the constant-force
+// velocity plus half the IBM force increments so far over rho), one vector
+// load per stencil point.
+template <typename T>
+struct alignas(4 * sizeof(T)) StageRec {
+    T rho, ux, uy, uz;
+};
+
 template <typename T>
 struct Params {
     int nx, ny, nz;
@@ -130,6 +139,7 @@ struct Params {
     // their force from ibm_f*, every other node the constant (fx, fy, fz)
     const uint32_t* tile_flag;
     const T* ibm_fx; const T* ibm_fy; const T* ibm_fz;
+    StageRec<T>* ibm_stage;     // stage records of the listed tiles (written by k_ibm_macr)
     int tiles_x, tiles_y;
 };
 
