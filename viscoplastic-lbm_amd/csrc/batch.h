@@ -39,6 +39,6 @@ int launch_batch_check(int q, int prec, int block, uint32_t grid, const void* jo
                        const uint16_t* unit_job, const uint32_t* n_jobs, uint32_t n_jobs_host, double* partials,
                        double* sums, void* stream);
 // rest state g_i = w_i (rho0 - 1) in deviation storage
-int launch_batch_fill_rest(int q, int prec, void* lat, uint32_t n, double rho0, void* stream);
+int launch_batch_fill_rest(int q, int prec, void* lat, int64_t stride, uint32_t n, double rho0, void* stream);
 
 }  // namespace vlbm

@@ -130,7 +130,7 @@ struct Layout {
 Layout layout_of(const vlbm_desc& d, int tbytes) {
     const size_t n = (size_t)d.nx * d.ny * d.nz;
     Layout L;
-    L.lat = align_up((size_t)d.q * n * d.precision);
+    L.lat = align_up((size_t)d.q * (size_t)plane_stride(n) * d.precision);
     L.uold = align_up(n * tbytes);
     L.omega = d.rheology == VLBM_POWERLAW ? align_up(n * tbytes) : 0;
     L.total = 2 * L.lat + 3 * L.uold + L.omega;
@@ -212,7 +212,7 @@ int admit(vlbm_batch* b, int id) {
     const Layout L = layout_of(j.d, b->tbytes);
     const uint32_t n = (uint32_t)((size_t)j.d.nx * j.d.ny * j.d.nz);
     char* base = b->arena + j.off;
-    if (launch_batch_fill_rest(j.d.q, j.d.precision, base, n, j.d.rho0, b->stream))
+    if (launch_batch_fill_rest(j.d.q, j.d.precision, base, plane_stride(n), n, j.d.rho0, b->stream))
         return check_launch(-1, "rest-state kernel");
     HIP_TRY(hipMemsetAsync(base + 2 * L.lat, 0, 3 * L.uold + L.omega, b->stream));
     int rc;

@@ -304,7 +304,7 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
     if (!h) return fail(VLBM_ERR_ALLOC, "host allocation failed");
     h->d = *d;
     h->n = (size_t)d->nx * d->ny * d->nz;
-    h->stride = (int64_t)h->n;
+    h->stride = plane_stride(h->n);
     h->sbytes = d->precision;
     h->tbytes = d->precision == VLBM_FP64 ? 8 : 4;
     rc = build_words(*d, h->words);
@@ -326,7 +326,7 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
                           d->nx >= 64;
     h->mom = d->layout == VLBM_LAYOUT_MOMENTS || (d->layout == VLBM_LAYOUT_AUTO && mom_auto);
     h->planes = h->mom ? kCoef : d->q;
-    const size_t lat_bytes = (size_t)h->planes * h->n * h->sbytes;
+    const size_t lat_bytes = (size_t)h->planes * (size_t)h->stride * h->sbytes;
     for (int k = 0; k < 2; k++) {
         if ((rc = dalloc(h, &h->lat[k], lat_bytes))) { release(h); return rc; }
         if (hipMemsetAsync(h->lat[k], 0, lat_bytes, h->stream) != hipSuccess) { release(h); return fail(VLBM_ERR_DEVICE, "hipMemset"); }
@@ -427,13 +427,23 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
 // Q-plane population buffer of a coefficient handle, allocated on first use.
 static int need_pops(vlbm_h h) {
     if (h->pops) return VLBM_OK;
-    return dalloc(h, &h->pops, (size_t)h->d.q * h->n * h->sbytes);
+    return dalloc(h, &h->pops, (size_t)h->d.q * (size_t)h->stride * h->sbytes);
 }
 
 int vlbm_set_omega(vlbm_h h, const double* omega) {
     if (!h || !omega) return fail(VLBM_ERR_INVALID, "handle or omega is NULL");
     if (!h->omega_state) return fail(VLBM_ERR_INVALID, "omega state exists for the power law only");
     return upload_field(h, h->omega_state, omega);
+}
+
+// Q population planes of the state (plane stride h->stride) <-> Q packed
+// planes of n doubles.
+static int convert_planes(vlbm_h h, void* state, double* dbl, int to_double) {
+    for (int i = 0; i < h->d.q; i++)
+        if (launch_convert(h->sbytes, (char*)state + (size_t)i * (size_t)h->stride * h->sbytes, dbl + (size_t)i * h->n,
+                           h->n, to_double, h->stream))
+            return -1;
+    return 0;
 }
 
 int vlbm_load_pops(vlbm_h h, const double* pop) {
@@ -455,7 +465,7 @@ int vlbm_load_pops(vlbm_h h, const double* pop) {
     HIP_TRY(hipMalloc((void**)&dbl, dev_in.size() * 8));
     if (hipMemcpyAsync(dbl, dev_in.data(), dev_in.size() * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess)
         rc = fail(VLBM_ERR_DEVICE, "population upload failed");
-    else if (launch_convert(h->sbytes, target, dbl, dev_in.size(), 0, h->stream))
+    else if (convert_planes(h, target, dbl, 0))
         rc = check_launch(-1, "conversion kernel");
     else if (hipStreamSynchronize(h->stream) != hipSuccess)
         rc = fail(VLBM_ERR_DEVICE, "sync failed");
@@ -494,7 +504,7 @@ int vlbm_get_pops(vlbm_h h, double* pop) {
     HIP_TRY(hipMalloc((void**)&dbl, total * 8));
     std::vector<double> host(total);
     int rc = VLBM_OK;
-    if (launch_convert(h->sbytes, (void*)source, dbl, total, 1, h->stream)) rc = check_launch(-1, "conversion kernel");
+    if (convert_planes(h, (void*)source, dbl, 1)) rc = check_launch(-1, "conversion kernel");
     else if (hipMemcpyAsync(host.data(), dbl, total * 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
              hipStreamSynchronize(h->stream) != hipSuccess)
         rc = fail(VLBM_ERR_DEVICE, "population download failed");

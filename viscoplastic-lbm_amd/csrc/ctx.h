@@ -82,7 +82,7 @@ inline void fill_params_desc(const vlbm_desc& d, const uint32_t region_word[27],
     p.n = (uint32_t)((size_t)d.nx * d.ny * d.nz);
     p.div_nx = make_fastdiv((uint32_t)d.nx);
     p.div_nxy = make_fastdiv((uint32_t)(d.nx * d.ny));
-    p.stride = (int64_t)p.n;
+    p.stride = plane_stride(p.n);
     for (int r = 0; r < 27; r++) p.region_word[r] = region_word[r];
     for (int k = 0; k < 8; k++) {
         p.ux_bc[k] = (T)d.ux_bc[k]; p.uy_bc[k] = (T)d.uy_bc[k];

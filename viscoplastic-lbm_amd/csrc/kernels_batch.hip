@@ -106,12 +106,12 @@ __global__ __launch_bounds__(64) void k_batch_job_sums(const BatchJob<T>* __rest
 }
 
 template <int Q, typename S>
-__global__ __launch_bounds__(kBlock) void k_batch_fill_rest(S* lat, uint32_t n, double rho0) {
+__global__ __launch_bounds__(kBlock) void k_batch_fill_rest(S* lat, int64_t stride, uint32_t n, double rho0) {
     using T = typename ComputeOf<S>::type;
     const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
     if (idx >= n) return;
 #pragma unroll
-    for (int i = 0; i < Q; i++) lat[(int64_t)i * n + idx] = (S)((T)weight<Q>(i) * ((T)rho0 - (T)1));
+    for (int i = 0; i < Q; i++) lat[(int64_t)i * stride + idx] = (S)((T)weight<Q>(i) * ((T)rho0 - (T)1));
 }
 
 template <int Q, typename S, int BLOCK>
@@ -181,10 +181,11 @@ int launch_batch_check(int q, int prec, int block, uint32_t grid, const void* jo
 #undef VLBM_CHECK
 }
 
-int launch_batch_fill_rest(int q, int prec, void* lat, uint32_t n, double rho0, void* stream) {
+int launch_batch_fill_rest(int q, int prec, void* lat, int64_t stride, uint32_t n, double rho0, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     const dim3 grid((n + kBlock - 1) / kBlock);
-#define VLBM_REST(QQ, SS) hipLaunchKernelGGL((k_batch_fill_rest<QQ, SS>), grid, dim3(kBlock), 0, st, (SS*)lat, n, rho0)
+#define VLBM_REST(QQ, SS) \
+    hipLaunchKernelGGL((k_batch_fill_rest<QQ, SS>), grid, dim3(kBlock), 0, st, (SS*)lat, stride, n, rho0)
     if (q == 19) {
         if (prec == 8) VLBM_REST(19, double); else if (prec == 4) VLBM_REST(19, float); else VLBM_REST(19, _Float16);
     } else {

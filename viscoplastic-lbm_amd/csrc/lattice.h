@@ -100,6 +100,15 @@ inline FastDiv make_fastdiv(uint32_t d) {
     return f;
 }
 
+// Elements between two planes of the state (population or coefficient
+// planes): the node count rounded up to 64, so every plane starts on a
+// 256-byte boundary (FP32). With planes packed back to back, node counts that
+// are not a multiple of 16 cost a quarter to a third of the step rate
+// (D3Q19 FP32 185x119x81: 23 GLUPS; the same lattice with NX rounded up to
+// 16: 32 GLUPS).
+constexpr size_t kPlaneAlign = 64;
+inline int64_t plane_stride(size_t n) { return (int64_t)((n + kPlaneAlign - 1) / kPlaneAlign * kPlaneAlign); }
+
 // Record of one node in the immersed boundary stage array: rho and the
 // velocity the interpolation reads, u + (f - f0) / 2 rho (the constant-force
 // velocity plus half the IBM force increments so far over rho), one vector
