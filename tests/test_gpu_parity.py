@@ -102,6 +102,9 @@ CASES_FP64 = [
                                   n_index=0.6, k_consistency=0.02, tau_clamp=[0.51, 2.0]), 200, 1e-4),
     ("circular_duct_ibb", dict(NX=18, NY=18, NZ=4, tau=0.8, FZ=1e-5, builder="circularDuctInterpBounceBack"),
      120, 0.0),
+    # NX dividing the 256-node block: the D3Q19 step pulls through the LDS row exchange (with the node map)
+    ("circular_duct_ibb_rows", dict(NX=16, NY=16, NZ=5, tau=0.8, FZ=1e-5, builder="circularDuctInterpBounceBack"),
+     120, 0.0),
     ("annular_duct_ibb_q27", dict(NX=22, NY=22, NZ=3, tau=0.8, FZ=1e-5, vel_set="D3Q27",
                                   builder="annularDuctInterpBounceBack", inner_rotation=0.002), 80, 0.0),
     # BGK collision (collisionSchemes/recursiveCollisionForceNoReg.txt) with each rheology. In Bingham
@@ -153,7 +156,8 @@ def test_fp64_matches_oracle(name, kw, steps, noise, layout):
 @pytest.mark.parametrize("name,kw,steps,noise", [c for c in CASES_FP64 if c[0] in
                                                   ("plates_bingham_c1", "square_duct_zouhe_c2", "closed_box_q27",
                                                    "plates_power_law_q27", "circular_duct_ibb",
-                                                   "square_duct_zouhe_bgk", "lid_driven_les")],
+                                                   "circular_duct_ibb_rows", "square_duct_zouhe_bgk",
+                                                   "lid_driven_les")],
                          ids=lambda v: v if isinstance(v, str) else "")
 @pytest.mark.parametrize("layout", LAYOUTS)
 def test_fp32_within_1e4_of_fp64_oracle(name, kw, steps, noise, layout):

@@ -36,8 +36,19 @@ struct Inst {
 
     template <int Q, int RHEO, bool MASK, bool IBM>
     static void step_pop(bool direct, const Params<T>& p, hipStream_t st) {
-        if (direct) hipLaunchKernelGGL((k_step<Q, S, RHEO, MASK, true, IBM>), grid_of(p.n), dim3(kBlock), 0, st, p);
-        else hipLaunchKernelGGL((k_step<Q, S, RHEO, MASK, false, IBM>), grid_of(p.n), dim3(kBlock), 0, st, p);
+        if (direct) {
+            hipLaunchKernelGGL((k_step<Q, S, RHEO, MASK, true, IBM>), grid_of(p.n), dim3(kBlock), 0, st, p);
+            return;
+        }
+        if constexpr (Q == 19) {
+            // x shift through LDS when rows never cross a block edge (NX divides the block)
+            if (kBlock % p.nx == 0) {
+                hipLaunchKernelGGL((k_step<Q, S, RHEO, MASK, false, IBM, kRepPop, kRepPop, true>), grid_of(p.n),
+                                   dim3(kBlock), 0, st, p);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_step<Q, S, RHEO, MASK, false, IBM>), grid_of(p.n), dim3(kBlock), 0, st, p);
     }
 
     template <int Q, int RHEO>
