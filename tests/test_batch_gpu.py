@@ -129,6 +129,25 @@ def test_autotuner_choice_is_cached(tmp_path):
     assert open(cache).read() == text     # found, not tuned again
 
 
+def test_autotuned_batch_keeps_each_job_s_steps_and_results():
+    """The autotuner times launch shapes on real chunks of the resident jobs
+    (14 candidates of block 128 and 256, two chunks of 10 steps each). The
+    residual checks run after every tuning chunk and the unit table follows
+    the block size, so each job stops at the same step with the same fields as
+    with a fixed launch shape: with a cap of 60 the jobs finish during the
+    tuning, with 400 they run through both block sizes."""
+    jobs = [spec(24, 20, 12, 1.0, 0.8, 5e-6), spec(30, 18, 10, 0.0, 0.7, 6e-6), spec(16, 22, 14, 3.0, 1.1, 8e-6)]
+    for cap in (60, 400):
+        print(f"case: autotuned vs fixed launch shape, cap {cap}", flush=True)
+        fixed, _ = run_batch(jobs, max_steps=cap)
+        tuned, cfg = run_batch(jobs, max_steps=cap, block=0, grid=0)
+        assert cfg["block"] in (128, 256) and cfg["grid"] > 0
+        for k in range(len(jobs)):
+            assert tuned[k][0]["steps"] == fixed[k][0]["steps"] == cap + 1, (k, tuned[k][0], fixed[k][0])
+            for f in ("rho", "ux", "uy", "uz"):
+                assert np.array_equal(tuned[k][1][f], fixed[k][1][f]), (k, cap, f)
+
+
 def test_converged_channel_matches_analytic_profile():
     """Newtonian and Bingham channels run to a residual of 1e-7: the x/z
     averaged u_x(y) follows the analytic profile (halfway walls)."""

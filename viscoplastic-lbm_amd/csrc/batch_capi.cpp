@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -69,6 +70,7 @@ struct vlbm_batch {
     uint16_t* d_unit_job = nullptr;            // job slot of every work unit
     size_t unit_cap = 0;                       // capacity of d_unit_job (bounded by the arena)
     bool units_dirty = true;                   // running set changed since the last upload
+    int units_block = 0;                       // block size the unit -> job table was built for
     size_t resident_nodes = 0;
     uint32_t* d_njobs = nullptr;
     double* d_partials = nullptr;
@@ -78,6 +80,9 @@ struct vlbm_batch {
     std::vector<uint32_t> prefix;              // host copy for the current block
     int K = 0, block = 0, grid = 0;
     bool tuned = false;
+    std::string tuned_key;                     // resident-node bucket the launch shape was chosen for
+    bool fixed_shape = false;                  // K, block and grid given by the caller
+    std::map<std::string, std::array<int, 3>> shapes;   // launch shapes chosen in this batch, by key
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_K = 0, graph_block = 0, graph_grid = 0, graph_rheo = -1;
@@ -177,12 +182,15 @@ int upload_table(vlbm_batch* b, const std::vector<int>& due) {
     }
     const uint32_t units = b->prefix[nj];
     if (units > b->unit_cap) return fail(VLBM_ERR_INVALID, "work units exceed the table capacity");
-    if (b->units_dirty && units) {
+    // the table depends on the running set and on the block size (units of
+    // block nodes); the autotuner changes the block between chunks
+    if ((b->units_dirty || b->units_block != b->block) && units) {
         std::vector<uint16_t> uj(units);
         for (size_t k = 0; k < nj; k++)
             for (uint32_t w = b->prefix[k]; w < b->prefix[k + 1]; w++) uj[w] = (uint16_t)k;
         HIP_TRY(hipMemcpyAsync(b->d_unit_job, uj.data(), (size_t)units * 2, hipMemcpyHostToDevice, b->stream));
         if (int rc = sync(b)) return rc;
+        b->units_block = b->block;
     }
     b->units_dirty = false;
     if (units > b->partials_cap) {
@@ -454,36 +462,58 @@ int autotune(vlbm_batch* b) {
     size_t nodes = 0;
     for (int id : b->running) nodes += (size_t)b->jobs[id].d.nx * b->jobs[id].d.ny * b->jobs[id].d.nz;
     const std::string key = tune_key(b, nodes);
+    b->tuned_key = key;
     int K = b->d.steps_per_launch, block = b->d.block, grid = b->d.grid;
     int cK, cb, cg;
-    if (cache_lookup(b, key, &cK, &cb, &cg)) {
+    const auto seen = b->shapes.find(key);
+    const bool known = seen != b->shapes.end();
+    if (known) { cK = seen->second[0]; cb = seen->second[1]; cg = seen->second[2]; }
+    if (known || cache_lookup(b, key, &cK, &cb, &cg)) {
+        b->shapes[key] = {cK, cb, cg};
         b->K = K ? K : cK; b->block = block ? block : cb; b->grid = grid ? grid : cg;
         b->tuned = true;
+        b->table_dirty = true;
         return VLBM_OK;
     }
     const std::vector<int> Ks = K ? std::vector<int>{K} : std::vector<int>{k_candidates(b->d.check_every).front()};
     const std::vector<int> blocks = block ? std::vector<int>{block} : std::vector<int>{128, 256};
-    const std::vector<int> grids = grid ? std::vector<int>{grid} : std::vector<int>{1024, 2048, 4096, 8192};
+    // up to one work unit per block for 2^24 resident nodes (surplus blocks exit at once)
+    const std::vector<int> grids = grid ? std::vector<int>{grid}
+                                        : std::vector<int>{1024, 2048, 4096, 8192, 16384, 32768, 65536};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    double best = 1e300;
+    double best = 0;
     int bK = Ks[0], bb = blocks[0], bg = grids[0];
     int rc = VLBM_OK;
-    // launch shape first with the largest K, then K for the best shape
+    // Each candidate runs a warm chunk (graph build) and a timed chunk of the
+    // resident jobs, with the jobs' residual checks after each chunk as in
+    // vlbm_batch_advance (a job's steps and results do not depend on the
+    // tuning); candidates compare by node updates per second.
+    auto resident = [&]() {
+        double n = 0;
+        for (int id : b->running) n += (double)b->jobs[id].d.nx * b->jobs[id].d.ny * b->jobs[id].d.nz;
+        return n;
+    };
     for (int bl : blocks) {
         for (int g : grids) {
+            if (b->running.empty()) break;
             b->K = Ks[0]; b->block = bl; b->grid = g;
             if ((rc = upload(b))) break;
-            if ((rc = run_chunk(b))) break;             // warm (graph build)
+            if ((rc = run_chunk(b))) break;
+            if ((rc = check_jobs(b))) break;
+            if (b->running.empty()) break;
+            if (b->table_dirty && (rc = upload(b))) break;
+            const double nodes = resident();
             (void)hipEventRecord(e0, b->stream);
             if ((rc = run_chunk(b))) break;
             (void)hipEventRecord(e1, b->stream);
             (void)hipEventSynchronize(e1);
             float ms = 0;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            const double per_step = ms / b->K;
-            if (per_step < best) { best = per_step; bb = bl; bg = g; }
+            if ((rc = check_jobs(b))) break;
+            const double rate = ms > 0 ? nodes * b->K / ms : 0;
+            if (rate > best) { best = rate; bb = bl; bg = g; }
         }
         if (rc) break;
     }
@@ -494,7 +524,16 @@ int autotune(vlbm_batch* b) {
     b->tuned = true;
     b->table_dirty = true;
     cache_store(b, key, bK, bb, bg);
+    b->shapes[key] = {bK, bb, bg};
     return VLBM_OK;
+}
+
+// The resident nodes are in another power-of-two bucket than the one the
+// launch shape was chosen for.
+bool retune_due(const vlbm_batch* b) {
+    size_t nodes = 0;
+    for (int id : b->running) nodes += (size_t)b->jobs[id].d.nx * b->jobs[id].d.ny * b->jobs[id].d.nz;
+    return tune_key(b, nodes) != b->tuned_key;
 }
 
 int need(vlbm_batch* b) {
@@ -571,6 +610,7 @@ int vlbm_batch_create(const vlbm_batch_desc* d, vlbm_batch_h* out) {
     b->block = d->block ? d->block : 256;
     b->grid = d->grid ? d->grid : 2048;
     b->tuned = d->steps_per_launch && d->block && d->grid;
+    b->fixed_shape = b->tuned;
     const int rc = upload(b);
     if (rc) { vlbm_batch_destroy(b); return rc; }
     *out = b;
@@ -632,10 +672,15 @@ int vlbm_batch_advance(vlbm_batch_h b, int* n_running, int* n_pending, int* n_fi
     for (const Job& j : b->jobs) finished_before += j.state >= VLBM_JOB_DONE;
     if ((rc = admit_pending(b))) return rc;
     if (!b->running.empty()) {
-        if (!b->tuned && (rc = autotune(b))) return rc;
-        if (b->table_dirty && (rc = upload(b))) return rc;
-        if ((rc = run_chunk(b))) return rc;
-        if ((rc = check_jobs(b))) return rc;
+        // (re)tune when the resident nodes enter another power-of-two bucket
+        if (!b->tuned || (!b->fixed_shape && retune_due(b))) {
+            if ((rc = autotune(b))) return rc;
+        }
+        if (!b->running.empty()) {               // tuning chunks may have finished every job
+            if (b->table_dirty && (rc = upload(b))) return rc;
+            if ((rc = run_chunk(b))) return rc;
+            if ((rc = check_jobs(b))) return rc;
+        }
         if ((rc = sync(b))) return rc;
     }
     int finished = 0;
