@@ -1781,6 +1781,29 @@ void or_collide_node(void* h, const double* fin, double fx, double fy, double fz
 // Direct access to the BC schemes on a lattice whose pop (post-collision) and
 // popAux (post-stream) arrays are set from the given buffers; the updated
 // popAux is returned in post_stream. Used by the boundary-scheme pin tests.
+// Test hook: replaces the node map (words [NZ][NY][NX]) and rebuilds the BC
+// list in the same z->y->x order, so boundary schemes can be placed on faces
+// no builder uses.
+void or_set_map(void* h, const uint32_t* words) {
+    OR_DISPATCH(h, {
+        for (size_t k = 0; k < L->map.size(); k++) L->map[k] = words[k];
+        L->bc_list.clear();
+        for (size_t k = 0; k < L->map.size(); k++)
+            if (w_used(L->map[k]) && w_scheme(L->map[k]) != BC_NULL) L->bc_list.push_back(k);
+    });
+}
+
+// Test hook: the interpolated bounce-back wall distance q of a link from
+// (x1, y1) towards (x2, y2) to the circle of radius r (kind 0: along x,
+// 1: along y, 2: diagonal of slope +1, 3: slope -1), as bc_interp_bb uses it.
+double or_ibb_q(int kind, double x1, double y1, double x2, double y2, double r) {
+    using L = Lattice<double>;
+    if (kind == 0) return L::q_a0(x1, y1, x2, y2, r);
+    if (kind == 1) return L::q_ainf(x1, y1, x2, y2, r);
+    if (kind == 2) return L::q_ap1(x1, y1, x2, y2, r);
+    return L::q_am1(x1, y1, x2, y2, r);
+}
+
 void or_apply_bc_raw(void* h, double* post_stream, const double* post_col) {
     OR_DISPATCH(h, {
         for (size_t k = 0; k < L->pop.size(); k++) { L->popAux[k] = post_stream[k]; L->pop[k] = post_col[k]; }

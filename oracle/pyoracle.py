@@ -94,6 +94,9 @@ def lib():
         L.or_set_omega.argtypes = [C.c_void_p, dp]
         L.or_collide_node.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
         L.or_apply_bc_raw.argtypes = [C.c_void_p, dp, dp]
+        L.or_set_map.argtypes = [C.c_void_p, P(C.c_uint32)]
+        L.or_ibb_q.restype = C.c_double
+        L.or_ibb_q.argtypes = [C.c_int] + [C.c_double] * 5
         ip = P(C.c_int)
         L.or_sphere_polar.restype = C.c_int
         L.or_sphere_polar.argtypes = [C.c_double, dp, C.c_double, C.c_double, C.c_int, dp, dp, C.c_int]
@@ -299,11 +302,22 @@ class Oracle:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, ux, uy, uz)]
         lib().or_ibm_set_macr(self.h, *[_p(a) for a in arrs])
 
+    def set_map(self, words):
+        """Test hook: replaces the node map ([NZ][NY][NX] words) and the BC list."""
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        lib().or_set_map(self.h, w.ctypes.data_as(C.POINTER(C.c_uint32)))
+
     def apply_bc_raw(self, post_stream, post_col):
         ps = np.ascontiguousarray(post_stream, dtype=np.float64).copy()
         pc = np.ascontiguousarray(post_col, dtype=np.float64)
         lib().or_apply_bc_raw(self.h, _p(ps), _p(pc))
         return ps
+
+
+def ibb_q(kind, x1, y1, x2, y2, r) -> float:
+    """Interpolated bounce-back wall distance of a link (kind 0: along x,
+    1: along y, 2: diagonal slope +1, 3: slope -1), as the oracle's BC uses it."""
+    return lib().or_ibb_q(int(kind), float(x1), float(y1), float(x2), float(y2), float(r))
 
 
 def max_threads() -> int:

@@ -362,13 +362,146 @@ def test_velocity_zou_he_back_face_reference_lines():
     assert np.allclose(j[:2], 0.0, atol=1e-14)
 
 
+DIR_BITS = 0x1F << 13
+
+
+def _zh_custom(builder, scheme, faces, uvals, rhovals, shape=(6, 5, 4)):
+    """Oracle lattice of builder whose node map gets Zou-He words (scheme, the
+    builder's own value indices) on the faces {(axis, side): dir}: no reference
+    builder places Zou-He on W/E, or pressure Zou-He on N/S."""
+    nz, ny, nx = shape
+    kw = dict(NX=nx, NY=ny, NZ=nz, builder=builder, ux_bc=uvals[0], uy_bc=uvals[1], uz_bc=uvals[2], rho_bc=rhovals)
+    cfg = base_cfg(**kw)
+    o = oracle_for(cfg)
+    mp = po.build_map(po.desc_from(cfg.to_desc())).copy()
+    src = [int(w) for w in mp.ravel() if word_fields(int(w))["scheme"] == scheme][0]
+    for (axis, side), d in faces.items():
+        sl = [slice(None)] * 3
+        sl[2 - axis] = 0 if side < 0 else mp.shape[2 - axis] - 1
+        if axis == 0:
+            sl[1] = slice(1, ny - 1)      # keep the builder's N/S rows apart from the W/E columns
+        mp[tuple(sl)] = (src & ~DIR_BITS) | (d << 13)
+    o.set_map(mp)
+    ps, pc = random_pair((19, nz + 1, ny, nx), 11)
+    ps = ps + weights(19)[:, None, None, None]
+    out = o.apply_bc_raw(ps, pc)
+    return mp, ps, out
+
+
 def test_velocity_zou_he_west_east_reference_lines():
-    """parallelPlatesZouHe has no W/E Zou-He, so use the textbook W/E rows of a
-    synthetic map: checked through the oracle's lattice with the couette
-    builder rotated is not available; instead pin W and E on inflow of
-    squareDuct? No reference builder puts velocity Zou-He on W/E, so these
-    rows stay parity-unpinned beyond the sum rule below."""
-    pytest.skip("no reference builder exercises velocity Zou-He on W/E faces")
+    """Velocity Zou-He on W (x = 0) and E (x = NX-1), placed by a custom map
+    (D3Q19_VelZouHe.cu:75-120). W: density (T + 2K)/(1 - u_x) and momentum
+    rho_w u_w, except that population 15 takes rho(u_x - u_y)/6 where the
+    textbook has u_z, which shifts the density by rho_w (u_z - u_y)/6 (zero for
+    u_y = u_z). E adds to the unknown populations in place and sums the W
+    face's known set K_W: density change rho_w (u_x + (u_z - u_y)/6) with
+    rho_w = (T + 2 K_W)/(1 + u_x), and x momentum change minus that."""
+    c = cvec(19)
+    for uyv, uzv in ((0.01, 0.01), (0.0, -0.015)):
+        ux, uy, uz = [0.03] * 8, [uyv] * 8, [uzv] * 8     # any value index of the face word
+        mp, ps, out = _zh_custom("couetteZouHe", 1, {(0, -1): 2, (0, 1): 3}, (ux, uy, uz), [1] * 8)
+        nx = mp.shape[2]
+        wf = word_fields(int(mp[2, 2, 0]))
+        assert wf["scheme"] == 1 and wf["dir"] == 2
+        uw = np.array([ux[wf["uxi"]], uy[wf["uyi"]], uz[wf["uzi"]]])
+        f_pre, f_post = ps[:, 2, 2, 0], out[:, 2, 2, 0]
+        tk = f_pre[c[:, 0] == 0].sum() + 2 * f_pre[c[:, 0] == -1].sum()
+        rho_w = tk / (1 - uw[0])
+        rho, j = _moments(f_post)
+        typo = rho_w * (uw[2] - uw[1]) / 6
+        assert np.isclose(rho, rho_w + typo, rtol=1e-13)
+        if uyv == uzv:
+            assert np.allclose(j, rho_w * uw, atol=1e-14)
+        # E face
+        wf = word_fields(int(mp[2, 2, nx - 1]))
+        assert wf["scheme"] == 1 and wf["dir"] == 3
+        uw = np.array([ux[wf["uxi"]], uy[wf["uyi"]], uz[wf["uzi"]]])
+        f_pre, f_post = ps[:, 2, 2, nx - 1], out[:, 2, 2, nx - 1]
+        kw_ = f_pre[c[:, 0] == 0].sum() + 2 * f_pre[c[:, 0] == -1].sum()
+        rho_w = kw_ / (1 + uw[0])
+        dm = rho_w * (uw[0] + (uw[2] - uw[1]) / 6)
+        assert np.isclose(f_post.sum() - f_pre.sum(), dm, rtol=1e-12, atol=1e-15)
+        assert np.isclose((f_post - f_pre) @ c[:, 0], -dm, rtol=1e-12, atol=1e-15)
+        # only the five c_x = -1 populations change
+        assert np.array_equal(f_post[c[:, 0] != -1], f_pre[c[:, 0] != -1])
+
+
+def test_pressure_zou_he_on_north_south_west_east():
+    """Pressure Zou-He on N/S (y faces) and W (x = 0), placed by a custom map
+    (D3Q19_PresZouHe.cu:27-98): node density rho_w, zero tangential momentum,
+    normal momentum s (T + 2K - rho_w). E (x = NX-1, :99-121) adds in place
+    to the W face's unknown set with u_x = -1 + (T + 2 K_W)/rho_w: density
+    change rho_w u_x and x momentum change minus that."""
+    c = cvec(19)
+    rho_bc = [1.0, 1.002, 0.998, 1, 1, 1, 1, 1]
+    mp, ps, out = _zh_custom("parallelPlatesZouHe", 3, {(1, -1): 1, (1, 1): 0}, ([0] * 8,) * 3, rho_bc)
+    ny = mp.shape[1]
+    for y, s, d in ((0, -1, 1), (ny - 1, 1, 0)):
+        wf = word_fields(int(mp[2, y, 1]))
+        assert wf["scheme"] == 3 and wf["dir"] == d
+        rw = rho_bc[wf["rhoi"]]
+        f_pre, f_post = ps[:, 2, y, 1], out[:, 2, y, 1]
+        tk = f_pre[c[:, 1] == 0].sum() + 2 * f_pre[c[:, 1] == s].sum()
+        rho, j = _moments(f_post)
+        assert np.isclose(rho, rw, rtol=1e-13)
+        assert np.isclose(j[1], s * (tk - rw), rtol=1e-12)
+        assert np.allclose(j[[0, 2]], 0.0, atol=1e-14)
+    mp, ps, out = _zh_custom("parallelPlatesZouHe", 3, {(0, -1): 2, (0, 1): 3}, ([0] * 8,) * 3, rho_bc)
+    nx = mp.shape[2]
+    wf = word_fields(int(mp[2, 2, 0]))
+    assert wf["scheme"] == 3 and wf["dir"] == 2
+    rw = rho_bc[wf["rhoi"]]
+    f_pre, f_post = ps[:, 2, 2, 0], out[:, 2, 2, 0]
+    tk = f_pre[c[:, 0] == 0].sum() + 2 * f_pre[c[:, 0] == -1].sum()
+    rho, j = _moments(f_post)
+    assert np.isclose(rho, rw, rtol=1e-13)
+    assert np.isclose(j[0], -(tk - rw), rtol=1e-12)
+    assert np.allclose(j[1:], 0.0, atol=1e-14)
+    wf = word_fields(int(mp[2, 2, nx - 1]))
+    assert wf["scheme"] == 3 and wf["dir"] == 3
+    rw = rho_bc[wf["rhoi"]]
+    f_pre, f_post = ps[:, 2, 2, nx - 1], out[:, 2, 2, nx - 1]
+    kw_ = f_pre[c[:, 0] == 0].sum() + 2 * f_pre[c[:, 0] == -1].sum()
+    dm = rw * (-1 + kw_ / rw)
+    assert np.isclose(f_post.sum() - f_pre.sum(), dm, rtol=1e-12)
+    assert np.isclose((f_post - f_pre) @ c[:, 0], -dm, rtol=1e-12)
+    assert np.array_equal(f_post[c[:, 0] != -1], f_pre[c[:, 0] != -1])
+
+
+def test_interpolated_bounce_back_wall_distance_is_the_circle_crossing():
+    """q of the interpolated bounce-back (interpolatedBounceBack.cu) equals
+    the fraction t in [0, 1] of the link from the fluid node (x1, y1) to its
+    neighbour (x2, y2) at which the segment meets the circle |p| = R, solved
+    here as the quadratic |p1 + t (p2 - p1)|^2 = R^2 (axis and diagonal links,
+    both walls of an annulus)."""
+    rng = np.random.default_rng(5)
+    links = {0: [(1, 0), (-1, 0)], 1: [(0, 1), (0, -1)], 2: [(1, 1), (-1, -1)], 3: [(1, -1), (-1, 1)]}
+    checked = 0
+    for _ in range(400):
+        R = rng.uniform(4.0, 20.0)
+        ang = rng.uniform(0, 2 * np.pi)
+        inside = rng.random() < 0.5            # node inside the circle (outer wall) or outside (inner wall)
+        r1 = R - rng.uniform(0.05, 0.95) if inside else R + rng.uniform(0.05, 0.95)
+        x1, y1 = r1 * np.cos(ang), r1 * np.sin(ang)
+        for kind, dirs in links.items():
+            for dx, dy in dirs:
+                x2, y2 = x1 - dx, y1 - dy
+                if (x2 * x2 + y2 * y2 > R * R) == (x1 * x1 + y1 * y1 > R * R):
+                    continue                   # the link does not cross the circle
+                a = dx * dx + dy * dy
+                b = -2 * (x1 * dx + y1 * dy)
+                cc = x1 * x1 + y1 * y1 - R * R
+                disc = np.sqrt(b * b - 4 * a * cc)
+                qq = -0.5 * (b + np.copysign(disc, b))   # stable roots qq / a and cc / qq
+                t = [r for r in (qq / a, cc / qq) if -1e-12 <= r <= 1 + 1e-12]
+                assert len(t) == 1
+                q = po.ibb_q(kind, x1, y1, x2, y2, R)
+                # the distance casts one difference to float before squaring, as
+                # the reference's dist2 (globalFunctions.h:158-162): ~1e-7 relative;
+                # a wrong root, sign or link length is off by far more
+                assert abs(q - t[0]) < 1e-6, (kind, dx, dy, x1, y1, R, q, t)
+                checked += 1
+    assert checked > 500
 
 
 def test_pressure_zou_he_faces_impose_density():
