@@ -82,8 +82,8 @@ def main():
                     help="run only a plain device copy of BYTES (counter calibration) and print its time")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the lattice step has no "
-              "multi-GPU path in this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the library steps one "
+              "lattice per GPU and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:

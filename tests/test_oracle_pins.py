@@ -152,7 +152,8 @@ def test_collision_power_law_omega(clamp):
         assert np.isclose(m[4], om, rtol=1e-9)
         exp, *_ = hermite_collision(q, f, F, om)
         assert np.allclose(out, exp, atol=1e-15)
-    # first step after init: omega 0 -> nominal K (Appendix A-4 of the survey)
+    # first step after init: the stored omega is 0, so the strain-rate estimate uses the
+    # consistency K as the viscosity (omega = 1 / (0.5 + 3 K))
     _, m = o.collide_node(feq(q, 1.0, [0.01, 0, 0]), omega_old=0.0)
     om0 = 1 / (0.5 + 3 * kc)
     assert np.isclose(m[4], min(max(om0, 0.5), 1 / 0.51) if clamp else om0)
