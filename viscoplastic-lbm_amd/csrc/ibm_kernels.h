@@ -175,32 +175,43 @@ __global__ __launch_bounds__(1024) void k_ibm_list_tiles(IbmParams P) {
 // ---------------------------------------------------------------------------
 // cell list
 // ---------------------------------------------------------------------------
-// cells whose spheres can touch a sphere of the given cell (distinct cells)
-__device__ __forceinline__ int neighbour_cells(const IbmParams& P, int cell, int out[27]) {
-    int cc[3];
-    cc[0] = cell % P.cells[0];
-    cc[1] = (cell / P.cells[0]) % P.cells[1];
-    cc[2] = cell / (P.cells[0] * P.cells[1]);
-    int v[3][3], nv[3];
+// The distinct cells whose spheres can touch a sphere of the given cell, as
+// one index range per axis: axis a visits lo[a] + k (mod the cell count when
+// periodic) for k < cnt[a]. Kept as ranges, not a list: an indexed list of
+// 27 cells would live in private (scratch) memory, whose every access waits
+// on a memory round trip in the one-workgroup contact stage.
+struct CellRange {
+    int lo[3], cnt[3];
+};
+
+__device__ __forceinline__ CellRange neighbour_range(const IbmParams& P, int cell) {
+    const int cc[3] = {cell % P.cells[0], (cell / P.cells[0]) % P.cells[1], cell / (P.cells[0] * P.cells[1])};
+    CellRange r;
+#pragma unroll
     for (int a = 0; a < 3; a++) {
         const int nc = P.cells[a];
-        nv[a] = 0;
         if (P.periodic[a] && nc <= 3) {
-            for (int k = 0; k < nc; k++) v[a][nv[a]++] = k;
+            r.lo[a] = 0;
+            r.cnt[a] = nc;
+        } else if (P.periodic[a]) {
+            r.lo[a] = cc[a] - 1;
+            r.cnt[a] = 3;
         } else {
-            for (int d = -1; d <= 1; d++) {
-                int k = cc[a] + d;
-                if (P.periodic[a]) k = (k + nc) % nc;
-                else if (k < 0 || k >= nc) continue;
-                v[a][nv[a]++] = k;
-            }
+            const int lo = cc[a] > 0 ? cc[a] - 1 : 0, hi = cc[a] + 1 < nc ? cc[a] + 1 : nc - 1;
+            r.lo[a] = lo;
+            r.cnt[a] = hi - lo + 1;
         }
     }
-    int n = 0;
-    for (int c = 0; c < nv[2]; c++)
-        for (int b = 0; b < nv[1]; b++)
-            for (int a = 0; a < nv[0]; a++) out[n++] = v[0][a] + P.cells[0] * (v[1][b] + P.cells[1] * v[2][c]);
-    return n;
+    return r;
+}
+
+// cell (k0, k1, k2) of a range, k_a < cnt[a]
+__device__ __forceinline__ uint64_t range_cell(const IbmParams& P, const CellRange& r, int k0, int k1, int k2) {
+    int k[3] = {r.lo[0] + k0, r.lo[1] + k1, r.lo[2] + k2};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (P.periodic[a]) k[a] = (k[a] + P.cells[a]) % P.cells[a];
+    return (uint64_t)(uint32_t)(k[0] + P.cells[0] * (k[1] + P.cells[1] * k[2]));
 }
 
 __device__ __forceinline__ int lower_bound_key(const uint64_t* keys, int n, uint64_t k) {
@@ -400,10 +411,9 @@ __device__ void sphere_contacts(const IbmParams& P, const uint64_t* keys, int i)
     IbmSphere& ci = P.sph[i];
     int cand[kMaxContacts];
     int nc = 0;
-    int cells[27];
-    const int ncell = neighbour_cells(P, ci.cell, cells);
-    for (int c = 0; c < ncell; c++) {
-        const uint64_t cell = (uint64_t)(uint32_t)cells[c];
+    const CellRange nb = neighbour_range(P, ci.cell);
+    for (int c = 0; c < nb.cnt[0] * nb.cnt[1] * nb.cnt[2]; c++) {
+        const uint64_t cell = range_cell(P, nb, c % nb.cnt[0], (c / nb.cnt[0]) % nb.cnt[1], c / (nb.cnt[0] * nb.cnt[1]));
         for (int k = lower_bound_key(keys, P.n_spheres, cell << 32); k < P.n_spheres && (keys[k] >> 32) == cell; k++) {
             const int j = (int)(keys[k] & 0xffffffffu);
             if (j <= i) continue;
@@ -447,10 +457,9 @@ __device__ void sphere_contact_sum(const IbmParams& P, const uint64_t* keys, int
     double M[3] = {cp.M[0], cp.M[1], cp.M[2]};
     int from_i[kMaxContacts], from_k[kMaxContacts];
     int nr = 0;
-    int cells[27];
-    const int ncell = neighbour_cells(P, cp.cell, cells);
-    for (int c = 0; c < ncell; c++) {
-        const uint64_t cell = (uint64_t)(uint32_t)cells[c];
+    const CellRange nb = neighbour_range(P, cp.cell);
+    for (int c = 0; c < nb.cnt[0] * nb.cnt[1] * nb.cnt[2]; c++) {
+        const uint64_t cell = range_cell(P, nb, c % nb.cnt[0], (c / nb.cnt[0]) % nb.cnt[1], c / (nb.cnt[0] * nb.cnt[1]));
         for (int k = lower_bound_key(keys, P.n_spheres, cell << 32); k < P.n_spheres && (keys[k] >> 32) == cell; k++) {
             const int i = (int)(keys[k] & 0xffffffffu);
             if (i >= p) continue;
