@@ -69,6 +69,10 @@ extern "C" {
                                                 pressure Zou-He outlet at z=NZ-1 (rho index 0) */
 #define VLBM_BUILDER_CIRCULAR_DUCT_IBB 12    /* circularDuctInterpBounceBack.cu */
 #define VLBM_BUILDER_ANNULAR_DUCT_IBB 13     /* annularDuctInterpBounceBack.cu */
+#define VLBM_BUILDER_RHEOMETER 14            /* parallelPlatesRheometer.cu: the circular duct, planes
+                                                z = 0 and NZ-1 unused, gpuSchSpecial on z = 1 (back,
+                                                at rest) and NZ-2 (front, outer_rotation); D3Q19,
+                                                population layout */
 
 /*
  * Simulation descriptor: the runtime form of the compile-time constants of

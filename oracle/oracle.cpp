@@ -257,19 +257,27 @@ R dist2d(R x1, R y1, R x2, R y2) {
 
 // circularDuctInterpBounceBack.cu:35-121 / annularDuctInterpBounceBack.cu.
 // Geometry evaluated in the reference's dfloat precision (double here).
+// parallelPlatesRheometer.cu:35-136 (rheometer): the circular duct with the
+// planes z = 0 and NZ-1 unused and gpuSchSpecial on z = 1 (BACK) and NZ-2
+// (FRONT), set before the wall classification, which may override it.
 template <typename R>
-void duct_word(const or_desc& d, int x, int y, Word& w, bool annular) {
+void duct_word(const or_desc& d, int x, int y, Word& w, bool annular, int z = -1) {
     const R Rout = (R)d.duct_outer_radius;
     const R xc = (R)(((d.nx - 1) / 2.0) + 0.5), yc = (R)(((d.ny - 1) / 2.0) + 0.5);
     const R xn = (R)(x + 0.5), yn = (R)(y + 0.5);
     const R dist = dist2d<R>(xn, yn, xc, yc);
     const int dirs[8][2] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}, {1, 1}, {-1, 1}, {-1, -1}, {1, -1}};
     if (!annular) {
-        if (dist > Rout) { w.used(false); return; }
+        const bool rheometer = z >= 0;
+        if (dist > Rout || (rheometer && (z == 0 || z == d.nz - 1))) { w.used(false); return; }
+        if (rheometer) {
+            if (z == 1) { w.scheme(SPECIAL); w.dir(B); }
+            else if (z == d.nz - 2) { w.scheme(SPECIAL); w.dir(F); }
+        }
         if (dist > (Rout - std::sqrt((float)2))) {
             w.scheme(INTERP_BB); w.dir(N);  // setIsInsideNodeInterpoBB(false) -> NORTH
         } else if (dist > (Rout - 2 * std::sqrt((float)2))) {
-            w.save(true); return;
+            w.save(true);                   // the unknown-population loop below runs for this ring too
         } else {
             return;
         }
@@ -312,6 +320,7 @@ uint32_t build_word(const or_desc& d, int x, int y, int z) {
     w.uxi(0); w.uyi(0); w.uzi(0); w.rhoi(0);
     if (d.builder == 12) { duct_word<double>(d, x, y, w, false); return w.m; }
     if (d.builder == 13) { duct_word<double>(d, x, y, w, true); return w.m; }
+    if (d.builder == 14) { duct_word<double>(d, x, y, w, false, z); return w.m; }
     if (short_chain_word(d.builder, x, y, z, d.nx, d.ny, d.nz, w)) return w.m;
     Act a[27];
     builder_actions(d.builder, a);
@@ -1040,6 +1049,40 @@ struct Lattice {
         }
     }
 
+    // gpuSchSpecial of the rheometer (parallelPlatesRheometer.cu:139-198):
+    // velocity bounce-back from the post-stream values of the node, wall
+    // density = the sum of all 19 post-stream values (the five whose source
+    // is an unused plane are the node's own post-collision values of the step
+    // before, left by the post-collision save in step()); FRONT rotates at the
+    // outer rotation, BACK is at rest. Other directions do nothing.
+    // Pinned by: tests derive every incoming population as moving-wall
+    // bounce-back with the rigid rotation omega (-y, x, 0) and compare.
+    void bc_special(int dir, int x, int y, int z) {
+        if (q != 19 || (dir != F && dir != B)) return;
+        const R xNode = x - (nx - 1) / 2.0, yNode = y - (ny - 1) / 2.0;
+        const R rr = std::sqrt(xNode * xNode + yNode * yNode);
+        const R c = xNode / (rr), s = yNode / (rr);
+        const R w_f = (R)d.outer_rotation, w_b = 0.0;
+        const R W1 = w[1], W2 = w[7];
+        R rho_w = PS(x, y, z, 0);
+        for (int i = 1; i < 19; i++) rho_w = rho_w + PS(x, y, z, i);
+        if (dir == F) {
+            const R ux_w = -w_f * rr * s, uy_w = w_f * rr * c, uz_w = 0;
+            PS(x, y, z, 6) = PS(x, y, z, 5) - 6 * rho_w * W1 * (uz_w);
+            PS(x, y, z, 10) = PS(x, y, z, 9) - 6 * rho_w * W2 * (uz_w + ux_w);
+            PS(x, y, z, 12) = PS(x, y, z, 11) - 6 * rho_w * W2 * (uz_w + uy_w);
+            PS(x, y, z, 15) = PS(x, y, z, 16) - 6 * rho_w * W2 * (uz_w - ux_w);
+            PS(x, y, z, 17) = PS(x, y, z, 18) - 6 * rho_w * W2 * (uz_w - uy_w);
+        } else {
+            const R ux_w = -w_b * rr * s, uy_w = w_b * rr * c, uz_w = 0;
+            PS(x, y, z, 5) = PS(x, y, z, 6) - 6 * rho_w * W1 * (-uz_w);
+            PS(x, y, z, 9) = PS(x, y, z, 10) - 6 * rho_w * W2 * (-uz_w - ux_w);
+            PS(x, y, z, 11) = PS(x, y, z, 12) - 6 * rho_w * W2 * (-uz_w - uy_w);
+            PS(x, y, z, 16) = PS(x, y, z, 15) - 6 * rho_w * W2 * (-uz_w + ux_w);
+            PS(x, y, z, 18) = PS(x, y, z, 17) - 6 * rho_w * W2 * (-uz_w + uy_w);
+        }
+    }
+
     void apply_bc(size_t idx) {
         const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((size_t)nx * ny));
         const uint32_t m = map[idx];
@@ -1050,6 +1093,7 @@ struct Lattice {
         case PRES_ZOUHE: bc_pres_zouhe(w_dir(m), x, y, z, (R)d.rho_bc[w_rhoi(m)]); break;
         case FREE_SLIP: bc_free_slip(w_dir(m), x, y, z); break;
         case INTERP_BB: bc_interp_bb(w_interp_bits(m), w_dir(m) == S, x, y, z); break;
+        case SPECIAL: bc_special(w_dir(m), x, y, z); break;
         default: break;
         }
     }

@@ -38,7 +38,8 @@ def test_defaults_follow_var_h():
 SIZES = [(4, 5, 6), (1, 3, 4), (3, 1, 5), (2, 2, 2), (7, 4, 1), (16, 9, 5)]
 
 
-@pytest.mark.parametrize("builder", [b for b in vlbm.BUILDERS if "Duct" not in b or "square" in b])
+@pytest.mark.parametrize("builder", [b for b in vlbm.BUILDERS
+                                     if ("Duct" not in b and "Rheometer" not in b) or "square" in b])
 @pytest.mark.parametrize("size", SIZES)
 def test_box_maps_bit_exact_vs_oracle(builder, size):
     nx, ny, nz = size
@@ -49,10 +50,14 @@ def test_box_maps_bit_exact_vs_oracle(builder, size):
     assert np.array_equal(mine, ref)
 
 
-@pytest.mark.parametrize("builder", ["circularDuctInterpBounceBack", "annularDuctInterpBounceBack"])
+@pytest.mark.parametrize("builder", ["circularDuctInterpBounceBack", "annularDuctInterpBounceBack",
+                                     "parallelPlatesRheometer"])
 @pytest.mark.parametrize("n", [12, 17, 24, 33])
-def test_duct_maps_bit_exact_vs_oracle(builder, n):
-    cfg = base_cfg(NX=n, NY=n, NZ=3, builder=builder)
+@pytest.mark.parametrize("nz", [3, 6])
+def test_duct_maps_bit_exact_vs_oracle(builder, n, nz):
+    """Circle masks, interpolated walls and, for the rheometer, the unused
+    end planes (z = 0, NZ-1) and the plate planes next to them."""
+    cfg = base_cfg(NX=n, NY=n, NZ=nz, builder=builder)
     d = cfg.to_desc()
     mine = vlbm.build_map(d)
     ref = po.build_map(po.desc_from(d))
@@ -69,6 +74,18 @@ def test_invalid_descriptors_are_rejected():
     with pytest.raises(vlbm.VlbmError) as e:
         vlbm.Lattice(d)
     assert e.value.code == 5
+
+
+@pytest.mark.parametrize("kw,code", [(dict(vel_set="D3Q27"), 5), (dict(layout="moments"), 5), (dict(NZ=2), 1)])
+def test_rheometer_refusals(kw, code):
+    """The plate condition is D3Q19 (parallelPlatesRheometer.cu defines no
+    D3Q27 lines), reads populations of the idle lattice (not kept by the
+    coefficient layout) and needs the two unused planes plus a fluid plane."""
+    cfg = dict(NX=12, NY=12, NZ=5, builder="parallelPlatesRheometer")
+    cfg.update(kw)
+    with pytest.raises(vlbm.VlbmError) as e:
+        vlbm.Lattice(base_cfg(**cfg).to_desc())
+    assert e.value.code == code and "rheometer" in str(e.value)
 
 
 def test_yaml_surface(tmp_path):

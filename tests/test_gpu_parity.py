@@ -105,6 +105,12 @@ CASES_FP64 = [
     # NX dividing the 256-node block: the D3Q19 step pulls through the LDS row exchange (with the node map)
     ("circular_duct_ibb_rows", dict(NX=16, NY=16, NZ=5, tau=0.8, FZ=1e-5, builder="circularDuctInterpBounceBack"),
      120, 0.0),
+    # parallel-plates rheometer (parallelPlatesRheometer.cu): the plate at z = NZ-2 turns at outer_rotation,
+    # the one at z = 1 rests; NX = 16 pulls through the row exchange, NX = 18 through the per-node gather
+    ("rheometer_rows", dict(NX=16, NY=16, NZ=6, tau=0.8, builder="parallelPlatesRheometer", outer_rotation=0.003),
+     120, 0.0),
+    ("rheometer_noisy", dict(NX=18, NY=18, NZ=5, tau=0.7, builder="parallelPlatesRheometer", outer_rotation=0.002),
+     100, 1e-3),
     ("annular_duct_ibb_q27", dict(NX=22, NY=22, NZ=3, tau=0.8, FZ=1e-5, vel_set="D3Q27",
                                   builder="annularDuctInterpBounceBack", inner_rotation=0.002), 80, 0.0),
     # BGK collision (collisionSchemes/recursiveCollisionForceNoReg.txt) with each rheology. In Bingham
@@ -138,6 +144,8 @@ LAYOUTS = ["populations", "moments"]
 def test_fp64_matches_oracle(name, kw, steps, noise, layout):
     if layout == "moments" and kw.get("collision") == "bgk":
         pytest.skip("the moment layout holds the regularized state only")
+    if layout == "moments" and kw["builder"] == "parallelPlatesRheometer":
+        pytest.skip("the rheometer's plates store populations; the moment layout is refused for it")
     announce(f"{name} [{layout}]")
     cfg = base_cfg(layout=layout, **kw)
     shape = (cfg.NZ, cfg.NY, cfg.NX)
@@ -156,13 +164,15 @@ def test_fp64_matches_oracle(name, kw, steps, noise, layout):
 @pytest.mark.parametrize("name,kw,steps,noise", [c for c in CASES_FP64 if c[0] in
                                                   ("plates_bingham_c1", "square_duct_zouhe_c2", "closed_box_q27",
                                                    "plates_power_law_q27", "circular_duct_ibb",
-                                                   "circular_duct_ibb_rows", "square_duct_zouhe_bgk",
+                                                   "circular_duct_ibb_rows", "rheometer_rows", "square_duct_zouhe_bgk",
                                                    "lid_driven_les")],
                          ids=lambda v: v if isinstance(v, str) else "")
 @pytest.mark.parametrize("layout", LAYOUTS)
 def test_fp32_within_1e4_of_fp64_oracle(name, kw, steps, noise, layout):
     if layout == "moments" and kw.get("collision") == "bgk":
         pytest.skip("the moment layout holds the regularized state only")
+    if layout == "moments" and kw["builder"] == "parallelPlatesRheometer":
+        pytest.skip("the rheometer's plates store populations; the moment layout is refused for it")
     announce(f"fp32 {name} [{layout}]")
     cfg = base_cfg(layout=layout, **kw)
     shape = (cfg.NZ, cfg.NY, cfg.NX)

@@ -542,6 +542,43 @@ def test_velocity_bounce_back_lid():
                 assert f_post[i] == f_pre[i]
 
 
+def test_rheometer_plates_are_moving_wall_bounce_back():
+    """parallelPlatesRheometer (gpuSchSpecial, parallelPlatesRheometer.cu:139-198):
+    at the plate nodes every incoming population is the streamed opposite plus
+    6 rho_w w_i c_i.u_w, rho_w the sum of all 19 streamed populations and u_w the
+    rigid rotation omega (-y, x, 0) about the duct axis at the front plate
+    (z = NZ-2), zero at the back plate (z = 1); nothing else changes."""
+    nz, ny, nx = 5, 12, 12
+    om = 0.004
+    cfg = base_cfg(NX=nx, NY=ny, NZ=nz, builder="parallelPlatesRheometer", outer_rotation=om)
+    o = oracle_for(cfg)
+    mp = po.build_map(po.desc_from(cfg.to_desc()))
+    ps, pc = random_pair((19, nz + 1, ny, nx), 5)
+    ps = ps + weights(19)[:, None, None, None]
+    out = o.apply_bc_raw(ps, pc)
+    c, w = cvec(19), weights(19)
+    seen = {1: 0, nz - 2: 0}
+    for z in (1, nz - 2):
+        for y in range(ny):
+            for x in range(nx):
+                wf = word_fields(int(mp[z, y, x]))
+                if not wf["used"] or wf["scheme"] != 7:
+                    continue
+                seen[z] += 1
+                f_pre, f_post = ps[:, z, y, x], out[:, z, y, x]
+                xn, yn = x - (nx - 1) / 2, y - (ny - 1) / 2
+                uw = om * np.array([-yn, xn, 0.0]) if z == nz - 2 else np.zeros(3)
+                inc = c[:, 2] == (-1 if z == nz - 2 else 1)
+                rho_w = f_pre.sum()
+                for i in range(19):
+                    if inc[i]:
+                        assert np.isclose(f_post[i], f_pre[opp(i)] + 6 * rho_w * w[i] * (c[i] @ uw),
+                                          rtol=1e-14, atol=1e-16), (z, y, x, i)
+                    else:
+                        assert f_post[i] == f_pre[i]
+    assert seen[1] > 20 and seen[nz - 2] == seen[1]
+
+
 # ---------------------------------------------------------------------------
 # builders (src/CUDA/boundaryConditionsBuilders/*.cu)
 # ---------------------------------------------------------------------------

@@ -66,7 +66,9 @@ uint32_t bc_word(int scheme, int dir, int uxi = 0, int uyi = 0, int uzi = 0, int
 }
 
 bool is_box_builder(int b) { return b >= VLBM_BUILDER_TAYLOR_GREEN && b <= VLBM_BUILDER_SQUARE_DUCT_ZOUHE; }
-bool is_duct_builder(int b) { return b == VLBM_BUILDER_CIRCULAR_DUCT_IBB || b == VLBM_BUILDER_ANNULAR_DUCT_IBB; }
+bool is_duct_builder(int b) {
+    return b == VLBM_BUILDER_CIRCULAR_DUCT_IBB || b == VLBM_BUILDER_ANNULAR_DUCT_IBB || b == VLBM_BUILDER_RHEOMETER;
+}
 
 // Node word of each box region for a builder.
 bool box_region_words(int builder, uint32_t out[27]) {
@@ -158,6 +160,7 @@ void duct_words(const vlbm_desc& d, std::vector<uint32_t>& out) {
                 scheme = kInterpBB; dir = dN; mark_unknowns = true;
             } else if (dist > (Ro - 2 * s2)) {
                 save = true;
+                mark_unknowns = true;   // the reference's unknown-population loop runs for this ring too
             } else if (annular && dist < Ri + s2) {
                 scheme = kInterpBB; dir = dS;
             } else if (annular && dist < Ri + 2 * s2) {
@@ -177,6 +180,21 @@ void duct_words(const vlbm_desc& d, std::vector<uint32_t>& out) {
         }
     out.resize((size_t)d.nx * d.ny * d.nz);
     for (int z = 0; z < d.nz; z++) std::memcpy(&out[(size_t)z * d.nx * d.ny], plane.data(), plane.size() * 4);
+    if (d.builder != VLBM_BUILDER_RHEOMETER) return;
+    // parallelPlatesRheometer.cu:70-89: planes z = 0 and NZ-1 unused; on z = 1
+    // (BACK) and NZ-2 (FRONT) gpuSchSpecial, unless the wall classification
+    // makes the node an interpolated bounce-back node
+    const size_t np = (size_t)d.nx * d.ny;
+    for (size_t k = 0; k < np; k++) {
+        out[k] = 0;
+        out[(size_t)(d.nz - 1) * np + k] = 0;
+        for (int z : {1, d.nz - 2}) {
+            uint32_t& w = out[(size_t)z * np + k];
+            if (!w_used(w) || w_scheme(w) == kInterpBB || z <= 0 || z >= d.nz - 1) continue;
+            const int dir = z == 1 ? dB : dF;
+            w = (w & ~((7u << kOffScheme) | (31u << kOffDir))) | ((uint32_t)kSpecial << kOffScheme) | ((uint32_t)dir << kOffDir);
+        }
+    }
 }
 
 int validate(const vlbm_desc* d) {
@@ -215,6 +233,12 @@ int validate(const vlbm_desc* d) {
                     return fail(VLBM_ERR_UNSUPPORTED, "Zou-He and velocity bounce-back are D3Q19-only");
             }
         }
+    }
+    if (d->builder == VLBM_BUILDER_RHEOMETER) {
+        if (d->q != 19) return fail(VLBM_ERR_UNSUPPORTED, "the rheometer's special boundary is D3Q19-only");
+        if (d->layout == VLBM_LAYOUT_MOMENTS)
+            return fail(VLBM_ERR_UNSUPPORTED, "the rheometer needs the population layout");
+        if (d->nz < 3) return fail(VLBM_ERR_INVALID, "the rheometer needs NZ >= 3");
     }
     if (is_duct_builder(d->builder)) {
         if (!(d->duct_outer_radius > 0)) return fail(VLBM_ERR_INVALID, "duct radius must be > 0");
@@ -397,8 +421,13 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
             break;
         }
         h->cur = 0;
-        if (launch_fill_feq(h->d.q, h->sbytes, h->lat[0], h->stride, (uint32_t)n, dev, dev + n, dev + 2 * n,
-                            dev + 3 * n, h->stream, h->mom ? kRepMom : kRepPop)) {
+        // both lattices: slots that no step writes (the rheometer's unused
+        // planes) hold the initial state, as both of the reference's buffers do
+        bool failed = false;
+        for (int k = 0; k < 2 && !failed; k++)
+            failed = launch_fill_feq(h->d.q, h->sbytes, h->lat[k], h->stride, (uint32_t)n, dev, dev + n, dev + 2 * n,
+                                     dev + 3 * n, h->stream, h->mom ? kRepMom : kRepPop) != 0;
+        if (failed) {
             rc = check_launch(-1, "equilibrium kernel");
             break;
         }
@@ -488,6 +517,11 @@ int vlbm_get_pops(vlbm_h h, double* pop) {
         int rc = h->mom ? need_pops(h) : VLBM_OK;
         if (rc) return rc;
         void* target = h->mom ? h->pops : h->lat[1 - h->cur];
+        // the rheometer's plates read the idle lattice in the next step: keep it
+        if (d.builder == VLBM_BUILDER_RHEOMETER) {
+            if (!h->scratch_pops) HIP_TRY(hipMalloc(&h->scratch_pops, (size_t)d.q * h->stride * h->sbytes));
+            target = h->scratch_pops;
+        }
         rc = with_params(h, [&](void* pv, auto& p) {
             p.src = h->lat[h->cur];
             p.dst = target;

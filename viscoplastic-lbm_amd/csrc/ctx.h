@@ -47,6 +47,7 @@ struct vlbm_ctx {
     int planes = 0;           // planes per lattice: Q, or 10 coefficients
     void* pops = nullptr;     // Q-plane population buffer of a coefficient handle:
                               // loaded post-stream state, or scratch of vlbm_get_pops
+    void* scratch_pops = nullptr;   // materialized populations of a rheometer handle (its idle lattice is read)
     bool direct_pops = false; // the current (direct) state is in pops
     long long steps = 0;
     uint32_t* d_mask = nullptr;
@@ -104,12 +105,17 @@ inline void fill_params_desc(const vlbm_desc& d, const uint32_t region_word[27],
     p.duct_R = (T)d.duct_outer_radius; p.duct_r = (T)d.duct_inner_radius;
     p.w_out = (T)d.outer_rotation; p.u_out = (T)d.outer_velocity;
     p.w_in = (T)d.inner_rotation; p.u_in = (T)d.inner_velocity;
+    p.plates = d.builder == VLBM_BUILDER_RHEOMETER ? 1 : 0;
 }
 
 template <typename T>
 inline void fill_params(const vlbm_ctx* h, Params<T>& p) {
     fill_params_desc(h->d, h->region_word, p);
     p.stride = h->stride;
+    p.held = h->lat[1 - h->cur];
+    // step 0 after init reads the post-stream state in place; step 1 (or a
+    // pass over the state it reads) holds the first boundary pass
+    p.first_bc = h->steps == 1 ? 1 : 0;
     p.mask = h->d_mask;
     p.omega_state = (T*)h->omega_state;
     p.m_rho = (T*)h->macr[0]; p.m_ux = (T*)h->macr[1]; p.m_uy = (T*)h->macr[2];
@@ -154,6 +160,7 @@ inline void release(vlbm_ctx* h) {
     if (!h) return;
     ibm_release(h);
     if (h->pops) (void)hipFree(h->pops);
+    if (h->scratch_pops) (void)hipFree(h->scratch_pops);
     for (void* p : h->lat) if (p) (void)hipFree(p);
     for (void* p : h->macr) if (p) (void)hipFree(p);
     for (void* p : h->uold) if (p) (void)hipFree(p);

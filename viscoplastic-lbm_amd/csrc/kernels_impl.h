@@ -286,7 +286,15 @@ __device__ void interp_bounce_back(T* g, const Params<T>& p, const Node<T>& nd, 
             } else {
                 // the reference reads x + c_i unwrapped; wrapping keeps degenerate
                 // (1-2 node wide) lattices in bounds and is the same node otherwise
-                const T fa = pc(j, kCX[i], kCY[i], kCZ[i]);
+                T fa = pc(j, kCX[i], kCY[i], kCZ[i]);
+                // rheometer: the reference refills an unused plane's populations
+                // toward the plates from its extra z plane (the rest state) in
+                // every step's stream, so only the first boundary pass sees the
+                // initial values there
+                if (p.plates && !p.first_bc) {
+                    const int za = (int)nd.z + kCZ[i];
+                    if (za == 0 || za == p.nz - 1) fa = (T)0;
+                }
                 g[i] = 2 * qd * fj + (1 - 2 * qd) * fa + (T)6 * df;
             }
         }
@@ -363,6 +371,11 @@ struct GlobalPC {
         if (IN == kRepPop) return load_pop<S, T>(p, i, node);
         return recon<Q, T, S>(i, (const S*)p.src + node, p.stride);
     }
+    // Population i of this node in the idle lattice (its post-collision value
+    // of the step before the last); read before the step overwrites it.
+    __device__ __forceinline__ T held(int i) const {
+        return ld<S, T>((const S*)p.held + (int64_t)i * p.stride + nd.idx);
+    }
 };
 
 // Gather (pull streaming) of the post-stream deviations: g_i(x) = g*_i(x - c_i).
@@ -437,6 +450,56 @@ __device__ __forceinline__ void load_direct(T* g, PC pc) {
     for (int i = 0; i < Q; i++) g[i] = pc(i, 0, 0, 0);
 }
 
+// gpuSchSpecial of the rheometer (parallelPlatesRheometer.cu:139-198), on
+// deviations (opposite populations have equal weights, so the reference's
+// lines carry over): velocity bounce-back from the node's own post-stream
+// values, the FRONT plate (z = NZ-2) turning at the outer rotation about the
+// duct axis, the BACK plate (z = 1) at rest; the wall density is the sum of
+// all 19 post-stream populations (the five from the unused plane are the
+// held values of plate_held).
+template <int Q, typename T>
+__device__ __forceinline__ void rheometer_plate(T* g, const Params<T>& p, const Node<T>& nd, bool front) {
+    const T xN = (T)((double)nd.x - (p.nx - 1) / 2.0), yN = (T)((double)nd.y - (p.ny - 1) / 2.0);
+    const T rr = sqrt(xN * xN + yN * yN);
+    const T c = xN / (rr), s = yN / (rr);
+    const T wr = front ? p.w_out : (T)0;
+    T rho_w = 0;
+#pragma unroll
+    for (int i = 0; i < 19; i++) rho_w = rho_w + (g[i] + (T)weight<19>(i));
+    const T W1 = (T)weight<19>(1), W2 = (T)weight<19>(7);
+    const T ux_w = -wr * rr * s, uy_w = wr * rr * c, uz_w = 0;
+    if (front) {
+        g[6] = g[5] - 6 * rho_w * W1 * (uz_w);
+        g[10] = g[9] - 6 * rho_w * W2 * (uz_w + ux_w);
+        g[12] = g[11] - 6 * rho_w * W2 * (uz_w + uy_w);
+        g[15] = g[16] - 6 * rho_w * W2 * (uz_w - ux_w);
+        g[17] = g[18] - 6 * rho_w * W2 * (uz_w - uy_w);
+    } else {
+        g[5] = g[6] - 6 * rho_w * W1 * (-uz_w);
+        g[9] = g[10] - 6 * rho_w * W2 * (-uz_w - ux_w);
+        g[11] = g[12] - 6 * rho_w * W2 * (-uz_w - uy_w);
+        g[16] = g[15] - 6 * rho_w * W2 * (-uz_w + ux_w);
+        g[18] = g[17] - 6 * rho_w * W2 * (-uz_w + uy_w);
+    }
+}
+
+// Rheometer plate nodes (z = 1, NZ-2; every one is a boundary node): no node
+// on the unused planes z = 0, NZ-1 collides or streams, and the reference's
+// stream step writes each boundary node's post-collision populations back
+// into the buffer it read (gpuMacrCollisionStream with the post-collision
+// save), so a population whose source is an unused plane arrives as the
+// node's own post-collision value of the step before. In the pull scheme
+// that value is the node's slot in the idle lattice.
+template <int Q, typename T, class PC>
+__device__ __forceinline__ void plate_held(T* g, const Params<T>& p, const Node<T>& nd, PC pc) {
+#pragma unroll
+    for (int i = 1; i < Q; i++) {
+        if (kCZ[i] == 0) continue;
+        const int zs = (int)nd.z - kCZ[i];
+        if (zs == 0 || zs == p.nz - 1) g[i] = pc.held(i);
+    }
+}
+
 // Boundary condition of the node's word on the gathered populations
 // (dispatch of src/CUDA/boundaryConditionsHandler.cu:25-79); pc gives the
 // post-collision populations of the node and its neighbours.
@@ -444,6 +507,7 @@ template <int Q, typename T, class PC>
 __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>& nd, uint32_t word, PC pc) {
     const int scheme = w_scheme(word);
     const int dir = w_dir(word);
+    if (p.plates) plate_held<Q, T>(g, p, nd, pc);
     if (scheme == kBounceBack) {
         // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
         if (dir >= dNW && w_geom(word)) return;
@@ -486,6 +550,8 @@ __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>
                       p.uz_bc[w_uzi(word)], p.rho_bc[w_rhoi(word)], !p.zouhe_textbook);
     } else if (scheme == kInterpBB) {
         interp_bounce_back<Q, T>(g, p, nd, w_interp(word), dir == dS, pc);
+    } else if (scheme == kSpecial) {
+        if (Q == 19 && (dir == dF || dir == dB)) rheometer_plate<Q, T>(g, p, nd, dir == dF);
     }
 }
 

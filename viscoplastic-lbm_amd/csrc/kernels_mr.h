@@ -37,6 +37,8 @@ struct TilePC {
         const T* q = sm + slot * (kCoef * SH::HP) + base + dy * SH::HX + dx;
         return recon<Q, T, T>(i, q, SH::HP);
     }
+    // the coefficient layout is refused for the rheometer, the one reader of held values
+    __device__ __forceinline__ T held(int) const { return (T)0; }
 };
 
 __device__ __forceinline__ int wrap_mod(int v, int n) {

@@ -126,6 +126,9 @@ struct Params {
     int64_t stride;             // elements between population planes
     const void* src;            // state read this step
     void* dst;                  // state written this step
+    const void* held;           // idle lattice (the step before the last; equals dst in a step)
+    int plates;                 // rheometer: the planes z = 0, NZ-1 are unused
+    int first_bc;               // the boundary pass of the first step after init
     const uint32_t* mask;       // node words (mask geometry only)
     uint32_t region_word[27];   // node word per box region (box geometry)
     T ux_bc[8], uy_bc[8], uz_bc[8], rho_bc[8];

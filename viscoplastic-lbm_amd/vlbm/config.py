@@ -53,6 +53,7 @@ BUILDERS = {
     "squareDuctZouHe": 11,
     "circularDuctInterpBounceBack": 12,
     "annularDuctInterpBounceBack": 13,
+    "parallelPlatesRheometer": 14,
 }
 
 PRECISIONS = {"double": _capi.FP64, "single": _capi.FP32, "half": _capi.FP16}
