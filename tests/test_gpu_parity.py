@@ -220,6 +220,32 @@ def test_state_roundtrip_and_on_demand_moments(layout):
     lat.close()
 
 
+def test_rheometer_steps_from_loaded_state():
+    """A rheometer state loaded by load_pops (the reference's checkpoint load
+    fills pop and popAux; the oracle's set_pops does the same) steps like the
+    oracle from the same loaded populations: the plates' held values and the
+    unused planes start from the loaded state."""
+    announce("rheometer from loaded state")
+    cfg = base_cfg(NX=18, NY=18, NZ=5, tau=0.7, builder="parallelPlatesRheometer", outer_rotation=0.002)
+    shape = (5, 18, 18)
+    src = po.Oracle(po.desc_from(cfg.to_desc()), 8)
+    src.init(np.ones(shape), *seeded_velocity(shape, (0, 0, 0), 1e-3, seed=21))
+    src.step(40)
+    pops = src.get_pops()
+    lat = vlbm.Lattice(cfg.to_desc())
+    lat.init()
+    lat.step(3)                     # the idle lattice holds other values before the load
+    lat.load_pops(pops)
+    orc = po.Oracle(po.desc_from(cfg.to_desc()), 8)
+    orc.init()                      # macroscopics of the unused planes, as lat.init() sets them
+    orc.set_pops(pops)
+    lat.step(60, save_last=True)
+    orc.step(60, save_last=True)
+    lat.sync()
+    assert_fp64_parity(lat, orc, 19)
+    lat.close()
+
+
 def test_reduce_matches_numpy():
     announce("reduce")
     cfg = base_cfg(NX=10, NY=9, NZ=8, tau=0.8, FX=1e-5, builder="parallelPlatesBounceBack_Y")

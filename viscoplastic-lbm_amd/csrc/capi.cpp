@@ -450,6 +450,7 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
     h->direct = true;
     h->direct_pops = false;
     h->steps = 0;
+    h->lbm_steps = 0;
     return ibm_reset_tiles(h);
 }
 
@@ -496,6 +497,11 @@ int vlbm_load_pops(vlbm_h h, const double* pop) {
         rc = fail(VLBM_ERR_DEVICE, "population upload failed");
     else if (convert_planes(h, target, dbl, 0))
         rc = check_launch(-1, "conversion kernel");
+    // the rheometer reads the idle lattice (held values, unused planes) in the
+    // first steps: both lattices take the loaded state, as both buffers do
+    // when the reference loads a checkpoint into pop and popAux
+    else if (d.builder == VLBM_BUILDER_RHEOMETER && convert_planes(h, h->lat[1], dbl, 0))
+        rc = check_launch(-1, "conversion kernel");
     else if (hipStreamSynchronize(h->stream) != hipSuccess)
         rc = fail(VLBM_ERR_DEVICE, "sync failed");
     (void)hipFree(dbl);
@@ -504,6 +510,7 @@ int vlbm_load_pops(vlbm_h h, const double* pop) {
     h->direct = true;
     h->direct_pops = h->mom;
     h->steps = 0;
+    h->lbm_steps = 0;
     return ibm_reset_tiles(h);
 }
 
@@ -585,6 +592,7 @@ static int enqueue_steps(vlbm_h h, int n_steps, int save_last, hipEvent_t* ev = 
         }
         if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 2], h->stream));
         h->steps++;
+        h->lbm_steps++;
     }
     return VLBM_OK;
 }

@@ -50,6 +50,7 @@ struct vlbm_ctx {
     void* scratch_pops = nullptr;   // materialized populations of a rheometer handle (its idle lattice is read)
     bool direct_pops = false; // the current (direct) state is in pops
     long long steps = 0;
+    long long lbm_steps = 0;  // LBM steps since init / load (the IBM stage alone does not count)
     uint32_t* d_mask = nullptr;
     std::vector<uint32_t> words;   // host map
     uint32_t region_word[27];
@@ -115,7 +116,7 @@ inline void fill_params(const vlbm_ctx* h, Params<T>& p) {
     p.held = h->lat[1 - h->cur];
     // step 0 after init reads the post-stream state in place; step 1 (or a
     // pass over the state it reads) holds the first boundary pass
-    p.first_bc = h->steps == 1 ? 1 : 0;
+    p.first_bc = h->lbm_steps == 1 ? 1 : 0;
     p.mask = h->d_mask;
     p.omega_state = (T*)h->omega_state;
     p.m_rho = (T*)h->macr[0]; p.m_ux = (T*)h->macr[1]; p.m_uy = (T*)h->macr[2];
