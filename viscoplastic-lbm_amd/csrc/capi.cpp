@@ -449,6 +449,8 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
     if (h->omega_state && (rc = upload_field(h, h->omega_state, zero.data()))) return rc;
     h->direct = true;
     h->direct_pops = false;
+    h->aux_pending = false;
+    h->resumed = false;
     h->steps = 0;
     h->lbm_steps = 0;
     return ibm_reset_tiles(h);
@@ -476,9 +478,94 @@ static int convert_planes(vlbm_h h, void* state, double* dbl, int to_double) {
     return 0;
 }
 
+// (pop, z) of the rheometer's plate slots whose source is an unused plane.
+static std::vector<std::pair<int, int>> plate_slots(const vlbm_desc& d) {
+    std::vector<std::pair<int, int>> out;
+    if (d.builder != VLBM_BUILDER_RHEOMETER) return out;
+    for (int i = 1; i < d.q; i++)
+        for (int z = 1; z <= d.nz - 2; z += (d.nz - 3 > 0 ? d.nz - 3 : 1)) {
+            const int zs = z - kCZ[i];
+            if (kCZ[i] != 0 && (zs == 0 || zs == d.nz - 1)) out.push_back({i, z});
+        }
+    return out;
+}
+
+static int upload_ref_pops(vlbm_h h, const double* pop, void* target);
+
+}  // extern "C"
+
+namespace vlbm {
+namespace detail {
+
+int plate_aux(vlbm_ctx* h, double* pop_aux) {
+    const vlbm_desc& d = h->d;
+    const auto slots = plate_slots(d);
+    // right after init both of the reference's buffers hold the initial state
+    if (slots.empty() || h->direct) return VLBM_OK;
+    const size_t plane = (size_t)d.nx * d.ny, total = (size_t)d.q * h->n;
+    double* dbl = nullptr;
+    HIP_TRY(hipMalloc((void**)&dbl, total * 8));
+    std::vector<double> host(total);
+    int rc = VLBM_OK;
+    if (convert_planes(h, h->lat[h->cur], dbl, 1)) rc = check_launch(-1, "conversion kernel");
+    else if (hipMemcpyAsync(host.data(), dbl, total * 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+             hipStreamSynchronize(h->stream) != hipSuccess)
+        rc = fail(VLBM_ERR_DEVICE, "population download failed");
+    (void)hipFree(dbl);
+    if (rc) return rc;
+    for (const auto& s : slots) {
+        const double w = pop_weight(d.q, s.first);
+        double* dst = pop_aux + (size_t)s.first * plane * (d.nz + 1) + (size_t)s.second * plane;
+        const double* src = host.data() + (size_t)s.first * h->n + (size_t)s.second * plane;
+        for (size_t k = 0; k < plane; k++) dst[k] = src[k] + w;
+    }
+    return VLBM_OK;
+}
+
+int plate_aux_stage(vlbm_ctx* h, const double* pop_aux) {
+    const vlbm_desc& d = h->d;
+    if (plate_slots(d).empty()) return VLBM_OK;
+    if (!h->scratch_pops) HIP_TRY(hipMalloc(&h->scratch_pops, (size_t)d.q * h->stride * h->sbytes));
+    int rc = upload_ref_pops(h, pop_aux, h->scratch_pops);
+    if (rc) return rc;
+    h->aux_pending = true;
+    return VLBM_OK;
+}
+
+}  // namespace detail
+}  // namespace vlbm
+
+extern "C" {
+
+// Reference-layout populations (Q, NZ+1, NY, NX) into Q deviation planes of target.
+static int upload_ref_pops(vlbm_h h, const double* pop, void* target) {
+    const vlbm_desc& d = h->d;
+    const size_t plane = (size_t)d.nx * d.ny;
+    std::vector<double> dev_in((size_t)d.q * h->n);
+    for (int i = 0; i < d.q; i++) {
+        const double w = pop_weight(d.q, i);
+        const double* srcp = pop + (size_t)i * plane * (d.nz + 1);   // skip the ghost plane
+        double* dstp = dev_in.data() + (size_t)i * h->n;
+        for (size_t k = 0; k < h->n; k++) dstp[k] = srcp[k] - w;
+    }
+    double* dbl = nullptr;
+    HIP_TRY(hipMalloc((void**)&dbl, dev_in.size() * 8));
+    int rc = VLBM_OK;
+    if (hipMemcpyAsync(dbl, dev_in.data(), dev_in.size() * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = fail(VLBM_ERR_DEVICE, "population upload failed");
+    else if (convert_planes(h, target, dbl, 0))
+        rc = check_launch(-1, "conversion kernel");
+    else if (hipStreamSynchronize(h->stream) != hipSuccess)
+        rc = fail(VLBM_ERR_DEVICE, "sync failed");
+    (void)hipFree(dbl);
+    return rc;
+}
+
 int vlbm_load_pops(vlbm_h h, const double* pop) {
     if (!h || !pop) return fail(VLBM_ERR_INVALID, "handle or populations NULL");
     const vlbm_desc& d = h->d;
+    h->aux_pending = false;
+    h->resumed = false;
     const size_t plane = (size_t)d.nx * d.ny;
     std::vector<double> dev_in((size_t)d.q * h->n);
     for (int i = 0; i < d.q; i++) {
@@ -585,6 +672,17 @@ static int enqueue_steps(vlbm_h h, int n_steps, int save_last, hipEvent_t* ev = 
         h->cur = other;
         h->direct = false;
         h->direct_pops = false;
+        if (h->aux_pending) {
+            // rheometer resumed from a checkpoint: the direct step has read the
+            // loaded state, and the idle lattice now takes popAux's plate slots
+            h->aux_pending = false;
+            const size_t plane = (size_t)d.nx * d.ny;
+            for (const auto& sl : plate_slots(d)) {
+                const size_t off = ((size_t)sl.first * (size_t)h->stride + (size_t)sl.second * plane) * h->sbytes;
+                HIP_TRY(hipMemcpyAsync((char*)h->lat[1 - h->cur] + off, (const char*)h->scratch_pops + off,
+                                       plane * h->sbytes, hipMemcpyDeviceToDevice, h->stream));
+            }
+        }
         if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], h->stream));
         if (ibm) {
             if ((rc = ibm_stage(h))) return rc;

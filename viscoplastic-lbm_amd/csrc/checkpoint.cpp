@@ -322,11 +322,14 @@ int vlbm_save_checkpoint(vlbm_h h, const char* path, const char* id, int step) {
     if ((rc = write_bytes(pre + "curr_step.bin", &s32, 4))) return rc;
     // populations: the post-stream state the next step collides; the
     // reference's popAux is overwritten by that step before it is read, and
-    // gets the same values here
+    // gets the same values here, except the rheometer's plate slots fed by an
+    // unused plane (no node streams into them; they hold the plate node's
+    // post-collision populations)
     {
         std::vector<double> pop((size_t)d.q * plane * (d.nz + 1));
         if ((rc = vlbm_get_pops(h, pop.data()))) return rc;
         if ((rc = write_values(pre + "pop.bin", pop.data(), pop.size(), tb))) return rc;
+        if ((rc = plate_aux(h, pop.data()))) return rc;
         if ((rc = write_values(pre + "popAux.bin", pop.data(), pop.size(), tb))) return rc;
     }
     // saved macroscopics (with IBM: those of the last stage on every node)
@@ -363,6 +366,11 @@ int vlbm_load_checkpoint(vlbm_h h, const char* path, const char* id, int* step) 
         std::vector<double> pop((size_t)d.q * plane * (d.nz + 1));
         if ((rc = read_values(pre + "pop.bin", pop.data(), pop.size(), tb))) return rc;
         if ((rc = vlbm_load_pops(h, pop.data()))) return rc;
+        if (d.builder == VLBM_BUILDER_RHEOMETER) {
+            if ((rc = read_values(pre + "popAux.bin", pop.data(), pop.size(), tb))) return rc;
+            if ((rc = plate_aux_stage(h, pop.data()))) return rc;
+            h->resumed = s32 > 0;
+        }
     }
     const size_t nb = h->n + 2 * (size_t)border * plane;
     std::vector<double> b(nb);

@@ -47,10 +47,13 @@ struct vlbm_ctx {
     int planes = 0;           // planes per lattice: Q, or 10 coefficients
     void* pops = nullptr;     // Q-plane population buffer of a coefficient handle:
                               // loaded post-stream state, or scratch of vlbm_get_pops
-    void* scratch_pops = nullptr;   // materialized populations of a rheometer handle (its idle lattice is read)
+    void* scratch_pops = nullptr;   // materialized populations of a rheometer handle (its idle lattice is read),
+                                    // or the staged popAux of a loaded checkpoint
+    bool aux_pending = false; // after the next (direct) step the idle lattice takes the staged plate slots
     bool direct_pops = false; // the current (direct) state is in pops
     long long steps = 0;
     long long lbm_steps = 0;  // LBM steps since init / load (the IBM stage alone does not count)
+    bool resumed = false;     // loaded from a checkpoint taken after at least one step
     uint32_t* d_mask = nullptr;
     std::vector<uint32_t> words;   // host map
     uint32_t region_word[27];
@@ -115,8 +118,9 @@ inline void fill_params(const vlbm_ctx* h, Params<T>& p) {
     p.stride = h->stride;
     p.held = h->lat[1 - h->cur];
     // step 0 after init reads the post-stream state in place; step 1 (or a
-    // pass over the state it reads) holds the first boundary pass
-    p.first_bc = h->lbm_steps == 1 ? 1 : 0;
+    // pass over the state it reads) holds the first boundary pass, unless the
+    // state comes from a checkpoint of a run that has had its first one
+    p.first_bc = (h->lbm_steps == 1 && !h->resumed) ? 1 : 0;
     p.mask = h->d_mask;
     p.omega_state = (T*)h->omega_state;
     p.m_rho = (T*)h->macr[0]; p.m_ux = (T*)h->macr[1]; p.m_uy = (T*)h->macr[2];
@@ -152,6 +156,11 @@ void ibm_release(vlbm_ctx* h);
 int ibm_stage(vlbm_ctx* h);        // one stage on the current state
 int ibm_fill_macr(vlbm_ctx* h);    // macroscopics outside the flagged tiles
 int ibm_reset_tiles(vlbm_ctx* h);  // no flagged tiles: every node takes the constant force
+
+// rheometer checkpoints (capi.cpp): the reference's popAux holds each plate
+// node's post-collision populations in the slots fed by an unused plane
+int plate_aux(vlbm_ctx* h, double* pop_aux);              // pop_aux (pop on entry) gets those slots
+int plate_aux_stage(vlbm_ctx* h, const double* pop_aux);  // after vlbm_load_pops
 
 // Representation and buffer of the current state.
 inline int state_rep(const vlbm_ctx* h) { return (h->mom && !h->direct_pops) ? 1 : 0; }
