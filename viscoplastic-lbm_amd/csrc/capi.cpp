@@ -566,33 +566,14 @@ int vlbm_load_pops(vlbm_h h, const double* pop) {
     const vlbm_desc& d = h->d;
     h->aux_pending = false;
     h->resumed = false;
-    const size_t plane = (size_t)d.nx * d.ny;
-    std::vector<double> dev_in((size_t)d.q * h->n);
-    for (int i = 0; i < d.q; i++) {
-        const double w = pop_weight(d.q, i);
-        const double* srcp = pop + (size_t)i * plane * (d.nz + 1);   // skip the ghost plane
-        double* dstp = dev_in.data() + (size_t)i * h->n;
-        for (size_t k = 0; k < h->n; k++) dstp[k] = srcp[k] - w;
-    }
     // a coefficient handle keeps the loaded populations until the first step
     int rc = h->mom ? need_pops(h) : VLBM_OK;
     if (rc) return rc;
-    void* target = h->mom ? h->pops : h->lat[0];
-    double* dbl = nullptr;
-    HIP_TRY(hipMalloc((void**)&dbl, dev_in.size() * 8));
-    if (hipMemcpyAsync(dbl, dev_in.data(), dev_in.size() * 8, hipMemcpyHostToDevice, h->stream) != hipSuccess)
-        rc = fail(VLBM_ERR_DEVICE, "population upload failed");
-    else if (convert_planes(h, target, dbl, 0))
-        rc = check_launch(-1, "conversion kernel");
+    if ((rc = upload_ref_pops(h, pop, h->mom ? h->pops : h->lat[0]))) return rc;
     // the rheometer reads the idle lattice (held values, unused planes) in the
     // first steps: both lattices take the loaded state, as both buffers do
     // when the reference loads a checkpoint into pop and popAux
-    else if (d.builder == VLBM_BUILDER_RHEOMETER && convert_planes(h, h->lat[1], dbl, 0))
-        rc = check_launch(-1, "conversion kernel");
-    else if (hipStreamSynchronize(h->stream) != hipSuccess)
-        rc = fail(VLBM_ERR_DEVICE, "sync failed");
-    (void)hipFree(dbl);
-    if (rc) return rc;
+    if (d.builder == VLBM_BUILDER_RHEOMETER && (rc = upload_ref_pops(h, pop, h->lat[1]))) return rc;
     h->cur = 0;
     h->direct = true;
     h->direct_pops = h->mom;
