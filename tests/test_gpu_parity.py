@@ -111,6 +111,8 @@ CASES_FP64 = [
      120, 0.0),
     ("rheometer_noisy", dict(NX=18, NY=18, NZ=5, tau=0.7, builder="parallelPlatesRheometer", outer_rotation=0.002),
      100, 1e-3),
+    ("rheometer_bingham", dict(NX=16, NY=16, NZ=7, tau=0.8, builder="parallelPlatesRheometer", outer_rotation=0.004,
+                               non_newtonian_model="BINGHAM", yield_stress=2e-5), 150, 0.0),
     ("annular_duct_ibb_q27", dict(NX=22, NY=22, NZ=3, tau=0.8, FZ=1e-5, vel_set="D3Q27",
                                   builder="annularDuctInterpBounceBack", inner_rotation=0.002), 80, 0.0),
     # BGK collision (collisionSchemes/recursiveCollisionForceNoReg.txt) with each rheology. In Bingham
@@ -164,7 +166,8 @@ def test_fp64_matches_oracle(name, kw, steps, noise, layout):
 @pytest.mark.parametrize("name,kw,steps,noise", [c for c in CASES_FP64 if c[0] in
                                                   ("plates_bingham_c1", "square_duct_zouhe_c2", "closed_box_q27",
                                                    "plates_power_law_q27", "circular_duct_ibb",
-                                                   "circular_duct_ibb_rows", "rheometer_rows", "square_duct_zouhe_bgk",
+                                                   "circular_duct_ibb_rows", "rheometer_rows", "rheometer_noisy",
+                                                   "rheometer_bingham", "square_duct_zouhe_bgk",
                                                    "lid_driven_les")],
                          ids=lambda v: v if isinstance(v, str) else "")
 @pytest.mark.parametrize("layout", LAYOUTS)
