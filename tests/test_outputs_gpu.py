@@ -102,6 +102,13 @@ def test_rheometer_checkpoint_resumes_exactly(tmp_path):
     b = vlbm.Lattice(cfg.to_desc())
     b.init()
     assert b.load_checkpoint(str(tmp_path), "rheo") == 70
+    # a checkpoint saved right after the load carries the same popAux
+    (tmp_path / "again").mkdir()
+    b.save_checkpoint(str(tmp_path / "again"), "rheo", 70)
+    for f in ("pop", "popAux"):
+        first = np.fromfile(tmp_path / "rheo" / "checkpoint" / f"0_{f}.bin", dtype="d")
+        again = np.fromfile(tmp_path / "again" / "rheo" / "checkpoint" / f"0_{f}.bin", dtype="d")
+        assert np.max(np.abs(first - again)) < 1e-15, f     # f - w + w round trip
     b.step(50)
     used = ((vlbm.build_map(a.desc) >> 21) & 1).astype(bool)
     pa, pb = a.get_pops()[:, :-1], b.get_pops()[:, :-1]

@@ -500,14 +500,16 @@ namespace detail {
 int plate_aux(vlbm_ctx* h, double* pop_aux) {
     const vlbm_desc& d = h->d;
     const auto slots = plate_slots(d);
-    // right after init both of the reference's buffers hold the initial state
-    if (slots.empty() || h->direct) return VLBM_OK;
+    // right after init both of the reference's buffers hold the initial state;
+    // right after a checkpoint load the staged popAux still holds the slots
+    if (slots.empty() || (h->direct && !h->aux_pending)) return VLBM_OK;
+    void* source = h->aux_pending ? h->scratch_pops : h->lat[h->cur];
     const size_t plane = (size_t)d.nx * d.ny, total = (size_t)d.q * h->n;
     double* dbl = nullptr;
     HIP_TRY(hipMalloc((void**)&dbl, total * 8));
     std::vector<double> host(total);
     int rc = VLBM_OK;
-    if (convert_planes(h, h->lat[h->cur], dbl, 1)) rc = check_launch(-1, "conversion kernel");
+    if (convert_planes(h, source, dbl, 1)) rc = check_launch(-1, "conversion kernel");
     else if (hipMemcpyAsync(host.data(), dbl, total * 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
              hipStreamSynchronize(h->stream) != hipSuccess)
         rc = fail(VLBM_ERR_DEVICE, "population download failed");
