@@ -249,6 +249,32 @@ def test_rheometer_steps_from_loaded_state():
     lat.close()
 
 
+def test_rheometer_steady_shear_is_rigid_rotation_times_gap_fraction():
+    """Physics of the rheometer, independent of the oracle: in steady creeping
+    flow near the axis the front plate (z = NZ-2, angular velocity omega)
+    drags the fluid against the resting back plate (z = 1) into plane Couette
+    flow, u_theta = omega r (z - 1) / (NZ - 3), with the walls at the plate
+    nodes. Checked in three radial bands well inside the cylinder."""
+    announce("rheometer steady shear")
+    n, nz, om = 48, 9, 0.002
+    cfg = base_cfg(NX=n, NY=n, NZ=nz, tau=0.8, builder="parallelPlatesRheometer", outer_rotation=om)
+    lat = vlbm.Lattice(cfg.to_desc())
+    lat.init()
+    lat.step(6000, save_last=True)
+    lat.sync()
+    m = lat.macroscopics()
+    lat.close()
+    y, x = np.mgrid[0:n, 0:n]
+    xn, yn = x - (n - 1) / 2, y - (n - 1) / 2
+    r = np.hypot(xn, yn)
+    ut = (-yn * m["ux"] + xn * m["uy"]) / np.where(r > 0, r, 1)
+    for lo, hi in ((2, 5), (5, 8), (8, 11)):
+        band = (r >= lo) & (r < hi)
+        for z in range(1, nz - 1):
+            phi = np.mean(ut[z][band] / (om * r[band]))
+            assert abs(phi - (z - 1) / (nz - 3)) < 1e-3, (lo, z, phi)
+
+
 def test_reduce_matches_numpy():
     announce("reduce")
     cfg = base_cfg(NX=10, NY=9, NZ=8, tau=0.8, FX=1e-5, builder="parallelPlatesBounceBack_Y")
