@@ -58,7 +58,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_check(const BatchJob<typename C
             const GlobalPC<Q, S, T, kRepPop> pc{p, nd};
             T g[Q];
             gather<Q, T>(g, pc);
-            if (w_used(word) && w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
+            if (w_used(word) && w_scheme(word) != kNull) apply_bc<Q, T, false>(g, p, nd, word, pc);
             const Moments<Q, 0, T> m = moments<Q, 0, T>(g, p, (T)0, const_force(p));
             const double ux = (double)m.ux, uy = (double)m.uy, uz = (double)m.uz;
             const double dx = ux - (double)J.uold[0][idx], dy = uy - (double)J.uold[1][idx], dz = uz - (double)J.uold[2][idx];

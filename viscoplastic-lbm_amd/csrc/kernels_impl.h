@@ -502,12 +502,17 @@ __device__ __forceinline__ void plate_held(T* g, const Params<T>& p, const Node<
 
 // Boundary condition of the node's word on the gathered populations
 // (dispatch of src/CUDA/boundaryConditionsHandler.cu:25-79); pc gives the
-// post-collision populations of the node and its neighbours.
-template <int Q, typename T, class PC>
+// post-collision populations of the node and its neighbours. MASK: the words
+// come from a per-node map (duct and rheometer builders). Box builders' words
+// never carry interpolated bounce-back or the rheometer's special scheme, so
+// their kernels are compiled without that code.
+template <int Q, typename T, bool MASK, class PC>
 __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>& nd, uint32_t word, PC pc) {
     const int scheme = w_scheme(word);
     const int dir = w_dir(word);
-    if (p.plates) plate_held<Q, T>(g, p, nd, pc);
+    if constexpr (MASK) {
+        if (p.plates) plate_held<Q, T>(g, p, nd, pc);
+    }
     if (scheme == kBounceBack) {
         // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
         if (dir >= dNW && w_geom(word)) return;
@@ -548,10 +553,12 @@ __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>
         if (Q == 19)
             zou_he<T>(g, dir, scheme == kPresZouHe, p.ux_bc[w_uxi(word)], p.uy_bc[w_uyi(word)],
                       p.uz_bc[w_uzi(word)], p.rho_bc[w_rhoi(word)], !p.zouhe_textbook);
-    } else if (scheme == kInterpBB) {
-        interp_bounce_back<Q, T>(g, p, nd, w_interp(word), dir == dS, pc);
-    } else if (scheme == kSpecial) {
-        if (Q == 19 && (dir == dF || dir == dB)) rheometer_plate<Q, T>(g, p, nd, dir == dF);
+    } else if constexpr (MASK) {
+        if (scheme == kInterpBB) {
+            interp_bounce_back<Q, T>(g, p, nd, w_interp(word), dir == dS, pc);
+        } else if (scheme == kSpecial) {
+            if (Q == 19 && (dir == dF || dir == dB)) rheometer_plate<Q, T>(g, p, nd, dir == dF);
+        }
     }
 }
 
@@ -789,7 +796,7 @@ __device__ __forceinline__ void step_node(const Params<typename ComputeOf<S>::ty
         load_direct<Q, T>(g, pc);
     } else {
         gather<Q, T>(g, pc);
-        if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
+        if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, pc);
     }
     collide_store<Q, S, RHEO, IBM, OUT>(p, idx, nd, g);
 }
@@ -805,7 +812,7 @@ __device__ __forceinline__ void step_node_rows(const Params<typename ComputeOf<S
     T g[Q];
     gather_rows<Q, S, T>(g, p, nd, sm);
     if (!in || !w_used(word)) return;
-    if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, GlobalPC<Q, S, T, kRepPop>{p, nd});
+    if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, GlobalPC<Q, S, T, kRepPop>{p, nd});
     collide_store<Q, S, RHEO, IBM, OUT>(p, idx, nd, g);
 }
 
@@ -844,7 +851,7 @@ __global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<
         load_direct<Q, T>(g, pc);
     } else {
         gather<Q, T>(g, pc);
-        if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
+        if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, pc);
     }
     if (p.aux_mode == kAuxMaterialize) {
         S* dst = (S*)p.dst;
@@ -891,7 +898,7 @@ __global__ __launch_bounds__(kBlock) void k_ibm_macr(const Params<typename Compu
             load_direct<Q, T>(g, pc);
         } else {
             gather<Q, T>(g, pc);
-            if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
+            if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, pc);
         }
         const Moments<Q, 0, T> m = moments<Q, 0, T>(g, p, (T)0, const_force(p));
         p.ibm_stage[idx] = StageRec<T>{m.rho, m.ux, m.uy, m.uz};

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(TileShape<T>::NT) void k_step_tile(const Params<T> 
                 const TilePC<Q, T> pc{sm, s_m | (s_0 << 4) | (s_p << 8), (ly + 1) * HX + (lx + 1)};
                 T g[Q];
                 gather<Q, T>(g, pc);
-                if (w_scheme(word) != kNull) apply_bc<Q, T>(g, p, nd, word, pc);
+                if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, pc);
                 collide_store<Q, T, RHEO, IBM, kRepMom>(p, nd.idx, nd, g);
             }
         }
