@@ -8,10 +8,18 @@ fused gather/BC/collide/store pass over the lattice.
 
 Prints one JSON line: MLUPS over the timed steps (wall clock, inputs resident
 in HBM), the roofline of the step kernel (algorithmic 2*Q*4 = 152 B/node over
-the HIP-event time of the same steps), and a CPU baseline (the oracle's
-OpenMP port of the reference per-node loop on the host cores, bounded sample).
+the HIP-event time of the same steps), a CPU baseline (the oracle's OpenMP
+port of the reference per-node loop on the host cores, bounded sample), and
+the serving sweep: the first N jobs of the seeded Poisson stream of D3Q19
+FP32 Bingham channels (20 jobs/s; Nx U[32,256], Ny U[32,128], Nz U[16,128],
+Bn U[0,5], tau U[0.55,1.5], F_x U[1e-6,1e-5]), each run to a velocity
+residual < 1e-6 or 50,000 steps (checked every 1,000 steps), batched through
+one persistent kernel and, for comparison, one at a time through the plain
+single-lattice path: jobs/s, aggregate MLUPS, p50/p99 arrival-to-result
+latency, converged count and the per-job error against the analytic plug
+profile.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--sweep-jobs N]
 """
 from __future__ import annotations
 
@@ -72,12 +80,50 @@ def cpu_baseline(init, target_s=15.0):
                       f"(oracle OpenMP loop, collapse(2) over z,y)"}
 
 
+SWEEP_RATE, SWEEP_SEED, SWEEP_RESID, SWEEP_MAX_STEPS, SWEEP_CHECK = 20.0, 7, 1e-6, 50000, 1000
+
+
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def serving_sweep(n_jobs):
+    """The first n_jobs of the seeded 2,000-job stream, batched and then one at
+    a time (same stop test and check cadence)."""
+    from vlbm.serve import make_jobs, sequential, sweep
+    jobs = make_jobs(n_jobs, SWEEP_RATE, SWEEP_SEED)
+    bat = sweep(jobs, SWEEP_RESID, SWEEP_MAX_STEPS, SWEEP_CHECK, realtime=True, hbm_budget=64 << 30,
+                max_jobs=256, max_resident_nodes=1 << 24)
+    progress(f"sweep: {n_jobs} jobs batched in {bat.seconds:.1f} s")
+    seq = sequential(jobs, SWEEP_RESID, SWEEP_MAX_STEPS, SWEEP_CHECK)
+    progress(f"sweep: {n_jobs} jobs one at a time in {seq.seconds:.1f} s")
+    b, q = bat.summary(), seq.summary()
+    return {
+        "workload": f"first {n_jobs} jobs of the seeded Poisson stream (rate {SWEEP_RATE:g}/s, seed {SWEEP_SEED}) "
+                    f"of D3Q19 FP32 Bingham channels, Nx U[32,256], Ny U[32,128], Nz U[16,128], Bn U[0,5], "
+                    f"tau U[0.55,1.5], F_x U[1e-6,1e-5]; each to residual < {SWEEP_RESID:g} or "
+                    f"{SWEEP_MAX_STEPS} steps, checked every {SWEEP_CHECK} steps",
+        "jobs": n_jobs,
+        "jobs_per_s": b["jobs_per_s"], "mlups": b["mlups"], "p50_s": b["p50_s"], "p99_s": b["p99_s"],
+        "converged": b["states"].get("done", 0), "capped": b["states"].get("max_steps", 0),
+        "failed": b["states"].get("failed", 0),
+        "profile_err_median": b["profile_err_median"], "profile_err_max": b["profile_err_max"],
+        "batch_config": bat.config,
+        "sequential": {"jobs_per_s": q["jobs_per_s"], "mlups": q["mlups"], "p50_s": q["p50_s"], "p99_s": q["p99_s"],
+                       "converged": q["states"].get("done", 0), "profile_err_median": q["profile_err_median"],
+                       "profile_err_max": q["profile_err_max"]},
+        "speedup_jobs_per_s": round(b["jobs_per_s"] / q["jobs_per_s"], 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-jobs", type=int, default=24,
+                    help="jobs of the serving sweep (0: no sweep); the first N of the seeded stream")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="run only a plain device copy of BYTES (counter calibration) and print its time")
     args = ap.parse_args()
@@ -134,9 +180,12 @@ def main():
         print("non-finite state after the timed steps", file=sys.stderr)
         sys.exit(1)
 
+    progress(f"{args.steps} timed steps: {kernel_ms:.5f} ms per step launch")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(init)
+        progress(f"cpu baseline: {cpu['value']} MLUPS")
+    sweep_out = serving_sweep(args.sweep_jobs) if args.sweep_jobs > 0 else None
 
     line = {
         "metric": METRIC,
@@ -168,6 +217,7 @@ def main():
             "algorithmic_bytes_per_node": 2 * Q * STORAGE_BYTES,
         },
         "cpu_baseline": cpu,
+        "sweep": sweep_out,
         "mean_rho": mean_rho,
     }
     print(json.dumps(line), flush=True)

@@ -118,6 +118,15 @@ const char* vlbm_last_error(void);
  * by gpuBuildBoundaryConditions (boundaryConditionsBuilder.h:18-19). Host only. */
 int vlbm_build_map(const vlbm_desc* d, uint32_t* out);
 
+/* Node map written by a builder outside the library: NX*NY*NZ words in the
+ * layout of vlbm_build_map, replacing the handle's map (the drop-in for a
+ * user-written gpuBuildBoundaryConditions, boundaryConditionsBuilder.h:18-19,
+ * whose words the step's dispatch reads, boundaryConditionsHandler.cu:25-79).
+ * Refused: the rheometer, a handle with the immersed boundary stage, the
+ * special scheme, Zou-He off the six faces, D3Q27 Zou-He / velocity
+ * bounce-back, interpolated bounce-back without a duct radius. */
+int vlbm_set_map(vlbm_h h, const uint32_t* words);
+
 /* Linear indices of the used BC nodes in z,y,x order
  * (BoundaryConditionsInfo::setupBoundaryConditionsInfo,
  * src/CUDA/structs/boundaryConditionsInfo.h:76-116). out may be NULL to query the count. */
@@ -131,8 +140,11 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
 /* Loads / reads populations in the reference layout: Q planes of (NZ+1)*NY*NX
  * doubles, x fastest (idxPop, src/CUDA/globalFunctions.h:143-147), holding the
  * post-streaming, post-BC state that the next step collides
- * (the pop array of src/CUDA/simCheckpoint.cu:216-318). The ghost plane z=NZ
- * is ignored on load and written as zeros. */
+ * (the pop array of src/CUDA/simCheckpoint.cu:216-318). vlbm_get_pops returns
+ * the reference's whole buffer: nodes that never collide hold what its push
+ * streaming leaves there, and the extra plane z=NZ what the last step pushed
+ * across the z ends (the rest state w_i before any step). The extra plane is
+ * ignored on load. */
 int vlbm_load_pops(vlbm_h h, const double* pop);
 int vlbm_get_pops(vlbm_h h, double* pop);
 

@@ -1778,6 +1778,11 @@ void or_get_pops(void* h, double* out) {
 void or_set_pops(void* h, const double* in) {
     OR_DISPATCH(h, for (size_t k = 0; k < L->pop.size(); k++) { L->pop[k] = in[k]; L->popAux[k] = in[k]; });
 }
+// Checkpoint load of both buffers (src/CUDA/simCheckpoint.cu:257-260 reads
+// pop.bin into pop and popAux.bin into popAux).
+void or_set_pops2(void* h, const double* pop, const double* aux) {
+    OR_DISPATCH(h, for (size_t k = 0; k < L->pop.size(); k++) { L->pop[k] = pop[k]; L->popAux[k] = aux[k]; });
+}
 
 void or_step(void* h, int nsteps, int save_last, int nthreads) {
     if (nthreads <= 0) nthreads = omp_get_max_threads();

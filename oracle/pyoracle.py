@@ -86,6 +86,7 @@ def lib():
         L.or_init.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_get_pops.argtypes = [C.c_void_p, dp]
         L.or_set_pops.argtypes = [C.c_void_p, dp]
+        L.or_set_pops2.argtypes = [C.c_void_p, dp, dp]
         L.or_step.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.or_time_steps.restype = C.c_double
         L.or_time_steps.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -188,9 +189,15 @@ class Oracle:
         lib().or_get_pops(self.h, _p(out))
         return out
 
-    def set_pops(self, pop):
+    def set_pops(self, pop, pop_aux=None):
+        """Both buffers take pop, or pop and pop_aux (a checkpoint load)."""
         pop = np.ascontiguousarray(pop, dtype=np.float64)
-        lib().or_set_pops(self.h, _p(pop))
+        if pop_aux is None:
+            lib().or_set_pops(self.h, _p(pop))
+        else:
+            aux = np.ascontiguousarray(pop_aux, dtype=np.float64)
+            assert aux.shape == pop.shape
+            lib().or_set_pops2(self.h, _p(pop), _p(aux))
 
     def set_omega(self, omega):
         lib().or_set_omega(self.h, _p(np.ascontiguousarray(omega, dtype=np.float64)))

@@ -9,6 +9,8 @@ autotuner cache are checked on small jobs.
 import numpy as np
 import pytest
 
+from helpers import rel_l2
+
 pytestmark = pytest.mark.gpu
 
 
@@ -190,3 +192,86 @@ def test_node_budget_limits_residency_and_keeps_results():
         for f in ("rho", "ux", "uy", "uz"):
             assert np.array_equal(res[f], free[k][1][f]), (k, f)
     b.close()
+
+
+def _drive(b, submits, shapes):
+    """Submits (desc, resid, max_steps) in order, advances until every job has
+    finished; returns {job: (info, result)}."""
+    ids = [b.submit(d, r, m) for d, r, m in submits]
+    out = {}
+    calls = 0
+    while len(out) < len(ids):
+        b.advance()
+        calls += 1
+        assert calls < 5000
+        for k, jid in enumerate(ids):
+            if k not in out:
+                inf = b.info(jid)
+                if inf["state"] in ("done", "max_steps", "failed"):
+                    out[k] = (inf, b.result(jid, shapes[k]))
+    return out
+
+
+def test_job_in_an_autotuned_batch_matches_the_oracle():
+    """One job of a three-job FP64 batch whose launch shape the autotuner picks
+    against the oracle (the reference restated on the CPU): the batch's
+    result after S steps holds the moments of the state the next step
+    collides, i.e. what the oracle saves during step S + 1. Bar: rel-L2 of u
+    and of rho - 1 (against the larger of the rho - 1 and u norms) <= 1e-10."""
+    import pyoracle as po
+    from vlbm import _capi
+    from vlbm.serve import Batch
+    print("case: batched job vs oracle", flush=True)
+    specs = [spec(24, 20, 12, 1.5, 0.8, 5e-6), spec(30, 18, 10, 0.0, 0.7, 6e-6), spec(16, 22, 14, 3.0, 1.1, 8e-6)]
+    b = Batch(check_every=20, steps_per_launch=0, block=0, grid=0, hbm_budget=1 << 30, precision=_capi.FP64)
+    out = _drive(b, [(s.desc(_capi.FP64), 0.0, 200) for s in specs], [(s.nz, s.ny, s.nx) for s in specs])
+    cfg = b.config()
+    b.close()
+    assert cfg["block"] in (128, 256) and cfg["grid"] > 0
+    for k, s in enumerate(specs):
+        inf, res = out[k]
+        assert inf["state"] == "max_steps" and inf["steps"] == 201
+        orc = po.Oracle(po.desc_from(s.desc(_capi.FP64)), 8)
+        orc.init()
+        orc.step(inf["steps"])
+        orc.step(1, save_last=True)
+        om = orc.macroscopics()
+        orc.close()
+        u_b, u_o = np.stack([res[f] for f in ("ux", "uy", "uz")]), np.stack([om[f] for f in ("ux", "uy", "uz")])
+        e_u = rel_l2(u_b, u_o)
+        # a periodic channel's density stays uniform (rho - 1 is round-off): rho - 1 is measured
+        # against the larger of its own norm and the velocity's
+        e_r = float(np.linalg.norm(res["rho"] - om["rho"]) / max(np.linalg.norm(om["rho"] - 1), np.linalg.norm(u_o)))
+        print(f"job {k}: u {e_u:.2e}  rho-1 {e_r:.2e}", flush=True)
+        assert e_u <= 1e-10 and e_r <= 1e-10
+
+
+def test_non_finite_or_negative_density_jobs_are_evicted():
+    """The reference stops a run whose mean density is negative
+    (src/CUDA/treatData.cpp:451-459); the batch retires such a job as failed at
+    its next check, and also one whose sums are not finite. Injected on the
+    host: a job whose initial density is NaN and one whose initial density is
+    -0.5 run beside two good jobs, which finish with the results they have
+    alone."""
+    import math
+    from vlbm.serve import Batch
+    print("case: eviction of failed jobs", flush=True)
+    good = [spec(24, 20, 12, 1.5, 0.8, 5e-6), spec(16, 22, 14, 3.0, 1.1, 8e-6)]
+    bad_nan, bad_neg = good[0].desc(), good[1].desc()
+    bad_nan.rho0 = math.nan
+    bad_neg.rho0 = -0.5
+    shapes = [(s.nz, s.ny, s.nx) for s in good]
+    kw = dict(check_every=20, steps_per_launch=10, block=256, grid=64, hbm_budget=1 << 30)
+    b = Batch(**kw)
+    out = _drive(b, [(good[0].desc(), 0.0, 100), (bad_nan, 0.0, 100), (good[1].desc(), 0.0, 100),
+                     (bad_neg, 0.0, 100)], [shapes[0], shapes[0], shapes[1], shapes[1]])
+    b.close()
+    for k in (1, 3):
+        inf = out[k][0]
+        assert inf["state"] == "failed" and inf["steps"] == 21, inf     # retired at its first check
+    assert math.isnan(out[1][0]["mean_rho"]) and out[3][0]["mean_rho"] < 0
+    alone = [_drive(Batch(**kw), [(s.desc(), 0.0, 100)], [shp])[0] for s, shp in zip(good, shapes)]
+    for k, a in ((0, alone[0]), (2, alone[1])):
+        assert out[k][0]["state"] == a[0]["state"] == "max_steps" and out[k][0]["steps"] == a[0]["steps"] == 101
+        for f in ("rho", "ux", "uy", "uz"):
+            assert np.array_equal(out[k][1][f], a[1][f]), (k, f)

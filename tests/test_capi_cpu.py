@@ -76,11 +76,14 @@ def test_invalid_descriptors_are_rejected():
     assert e.value.code == 5
 
 
-@pytest.mark.parametrize("kw,code", [(dict(vel_set="D3Q27"), 5), (dict(layout="moments"), 5), (dict(NZ=2), 1)])
+@pytest.mark.parametrize("kw,code", [(dict(vel_set="D3Q27"), 5), (dict(layout="moments"), 5), (dict(NZ=2), 1),
+                                     (dict(NX=17, NY=17), 1)])
 def test_rheometer_refusals(kw, code):
     """The plate condition is D3Q19 (parallelPlatesRheometer.cu defines no
     D3Q27 lines), reads populations of the idle lattice (not kept by the
-    coefficient layout) and needs the two unused planes plus a fluid plane."""
+    coefficient layout) and needs the two unused planes plus a fluid plane.
+    An odd x odd cross-section puts a plate node on the axis, where the
+    reference's wall velocity is 0/0."""
     cfg = dict(NX=12, NY=12, NZ=5, builder="parallelPlatesRheometer")
     cfg.update(kw)
     with pytest.raises(vlbm.VlbmError) as e:

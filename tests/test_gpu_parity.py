@@ -1,7 +1,7 @@
 """GPU parity: libvlbm.so (HIP, gfx950) against the CPU oracle.
 
 Bars: FP64 GPU vs FP64 oracle, rel-L2 of (f - w) <= 1e-10 and of u <= 1e-10;
-FP32 GPU vs FP64 oracle, rel-L2 of rho and u <= 1e-4; FP16 deviations vs FP64
+FP32 GPU vs FP64 oracle, rel-L2 of f - w, rho - 1 and u <= 1e-4; FP16 deviations vs FP64
 oracle <= 2e-2 on u (stated per test). Masks / BC lists bit-exact.
 """
 import sys
@@ -83,6 +83,11 @@ CASES_FP64 = [
     ("square_duct_zouhe_c2", dict(NX=12, NY=10, NZ=16, tau=0.8, builder="squareDuctZouHe",
                                   non_newtonian_model="BINGHAM", yield_stress=1e-5,
                                   uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0]), 150, 1e-4),
+    # the benchmarked kernel: NX = 16 divides the 256-node block, so the step pulls through the LDS row
+    # exchange with the box map, as the 256^3 workload does
+    ("square_duct_zouhe_c2_rows", dict(NX=16, NY=16, NZ=24, tau=0.8, builder="squareDuctZouHe",
+                                       non_newtonian_model="BINGHAM", yield_stress=1e-5,
+                                       uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0]), 150, 1e-4),
     ("plates_zouhe", dict(NX=6, NY=8, NZ=10, tau=0.8, builder="parallelPlatesZouHe",
                           rho_bc=[1, 1.001, 0.999, 1, 1, 1, 1, 1]), 120, 1e-3),
     ("lid_driven", dict(NX=9, NY=9, NZ=5, tau=0.7, builder="lidDrivenBounceBack", u_max=0.05), 120, 0.0),
@@ -164,7 +169,8 @@ def test_fp64_matches_oracle(name, kw, steps, noise, layout):
 
 
 @pytest.mark.parametrize("name,kw,steps,noise", [c for c in CASES_FP64 if c[0] in
-                                                  ("plates_bingham_c1", "square_duct_zouhe_c2", "closed_box_q27",
+                                                  ("plates_bingham_c1", "square_duct_zouhe_c2",
+                                                   "square_duct_zouhe_c2_rows", "closed_box_q27",
                                                    "plates_power_law_q27", "circular_duct_ibb",
                                                    "circular_duct_ibb_rows", "rheometer_rows", "rheometer_noisy",
                                                    "rheometer_bingham", "square_duct_zouhe_bgk",
@@ -184,11 +190,22 @@ def test_fp32_within_1e4_of_fp64_oracle(name, kw, steps, noise, layout):
         u0 = (0.0, 0.0, 0.01) if "zouhe_c2" in name else (0.0, 0.0, 0.0)
         init = (np.ones(shape),) + seeded_velocity(shape, u0, noise, seed=7)
     lat, orc = run_both(cfg, steps, init, prec_gpu=vlbm.FP32)
+    q = 19 if cfg.vel_set == "D3Q19" else 27
+    used = used_mask(lat)
+    gp, op = pops_dev(lat.get_pops(), q), pops_dev(orc.get_pops(), q)
+    # the signal is in the deviations: f - w, and rho - 1 = sum_i (f_i - w_i) (rho itself is ~1, so a
+    # bar on rho would pass a 10 % error in rho - 1); rho - 1 is measured against the larger of its own
+    # norm and the velocity's, since a periodic channel's density stays uniform (rho - 1 is round-off
+    # there); measured 1e-7 to 4e-6 on these cases
     gm, om = lat.macroscopics(), orc.macroscopics()
-    assert rel_l2(gm["rho"], om["rho"]) <= 1e-4
     u_g = np.stack([gm["ux"], gm["uy"], gm["uz"]])
     u_o = np.stack([om["ux"], om["uy"], om["uz"]])
-    assert rel_l2(u_g, u_o) <= 1e-4
+    e_u = rel_l2(u_g[:, used], u_o[:, used])
+    e_pop = rel_l2(gp[:, used], op[:, used])
+    dr_g, dr_o = gp.sum(0)[used], op.sum(0)[used]
+    e_dr = float(np.linalg.norm(dr_g - dr_o) / max(np.linalg.norm(dr_o), np.linalg.norm(u_o[:, used])))
+    print(f"fp32 vs fp64 oracle: f-w {e_pop:.2e}  rho-1 {e_dr:.2e}  u {e_u:.2e}", flush=True)
+    assert e_pop <= 1e-4 and e_dr <= 1e-4 and e_u <= 1e-4
     lat.close()
 
 
@@ -323,23 +340,91 @@ def c2_config(precision):
                     non_newtonian_model="BINGHAM", yield_stress=1e-5, uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0])
 
 
-def test_c2_fp32_vs_fp64_full_size():
-    """At 256^3 the FP32 path stays within 1e-4 rel-L2 (rho, u) of the FP64 GPU
-    path, which matches the oracle at small sizes (test_fp64_matches_oracle)."""
-    announce("c2 256^3 fp32 vs fp64, 300 steps")
+def c2_init():
     shape = (256, 256, 256)
-    init = (np.ones(shape),) + seeded_velocity(shape, (0, 0, 0.01), 1e-4, seed=12345)
+    return (np.ones(shape),) + seeded_velocity(shape, (0, 0, 0.01), 1e-4, seed=12345)
+
+
+def dev_sums(pops, q):
+    """rho - 1 per node from populations f (sum of f - w, in double)."""
+    w = weights(q)
+    out = np.zeros(pops.shape[1:])
+    for i in range(q):
+        out += pops[i] - w[i]
+    return out
+
+
+def test_c2_fp32_vs_fp64_oracle_full_size():
+    """The benchmarked kernel at the benchmarked size: 256^3 D3Q19 FP32 Bingham
+    duct (box map, LDS row exchange) against the FP64 oracle after 100 steps
+    from the same seeded start: rel-L2 of u and of rho - 1 <= 1e-4."""
+    announce("c2 256^3 fp32 vs fp64 oracle, 100 steps")
+    init = c2_init()
+    lat = vlbm.Lattice(c2_config("single").to_desc())
+    lat.init(*init)
+    lat.step(100, save_last=True)
+    lat.sync()
+    gm = lat.macroscopics()
+    dr_g = dev_sums(lat.get_pops()[:, :-1], 19)
+    lat.close()
+    orc = po.Oracle(po.desc_from(c2_config("double").to_desc()), 8)
+    orc.init(*init)
+    orc.step(100, save_last=True)
+    om = orc.macroscopics()
+    dr_o = dev_sums(orc.get_pops()[:, :-1], 19)
+    orc.close()
+    e_u = rel_l2(np.stack([gm[k] for k in ("ux", "uy", "uz")]), np.stack([om[k] for k in ("ux", "uy", "uz")]))
+    e_dr = rel_l2(dr_g, dr_o)
+    print(f"256^3 fp32 vs fp64 oracle after 100 steps: u {e_u:.2e}  rho-1 {e_dr:.2e}", flush=True)
+    assert e_u <= 1e-4 and e_dr <= 1e-4
+
+
+def test_c2_fp32_vs_fp64_full_size():
+    """The workload's full run, 5,000 steps at 256^3: the FP32 path stays within
+    1e-4 rel-L2 of u and rho of the FP64 path, which matches the oracle at
+    small sizes (test_fp64_matches_oracle) and at this size for 100 steps.
+    rho - 1 (the pressure field, ~1e-3 here) carries FP32 round-off of the
+    mass balance: measured 1.6e-4 after 5,000 steps, bar 5e-4;
+    test_fp32_density_deviation_within_the_reference_single_precision shows
+    the reference's own FP32 build does worse on it."""
+    announce("c2 256^3 fp32 vs fp64, 5000 steps")
+    init = c2_init()
     out = {}
     for prec in ("single", "double"):
         lat = vlbm.Lattice(c2_config(prec).to_desc())
         lat.init(*init)
-        lat.step(300, save_last=True)
-        out[prec] = lat.macroscopics()
+        lat.step(5000, save_last=True)
+        lat.sync()
+        m = lat.macroscopics()
+        out[prec] = (np.stack([m[k] for k in ("ux", "uy", "uz")]), dev_sums(lat.get_pops()[:, :-1], 19))
         lat.close()
-    assert rel_l2(out["single"]["rho"], out["double"]["rho"]) <= 1e-4
-    u32 = np.stack([out["single"][k] for k in ("ux", "uy", "uz")])
-    u64 = np.stack([out["double"][k] for k in ("ux", "uy", "uz")])
-    assert rel_l2(u32, u64) <= 1e-4
+    e_u = rel_l2(out["single"][0], out["double"][0])
+    e_dr = rel_l2(out["single"][1], out["double"][1])
+    e_rho = rel_l2(1 + out["single"][1], 1 + out["double"][1])
+    print(f"256^3 fp32 vs fp64 after 5000 steps: u {e_u:.2e}  rho {e_rho:.2e}  rho-1 {e_dr:.2e} "
+          f"(rms rho-1 {np.sqrt(np.mean(out['double'][1] ** 2)):.2e})", flush=True)
+    assert e_u <= 1e-4 and e_rho <= 1e-4 and e_dr <= 5e-4
+
+
+def test_fp32_density_deviation_within_the_reference_single_precision():
+    """rho - 1 of the FP32 path against the FP64 oracle is no worse than the
+    reference's own FP32 build (the oracle in float storage, which keeps f,
+    not f - w, in single precision) on the benchmarked duct kernel."""
+    announce("fp32 rho-1 vs the reference's FP32 build")
+    name, kw, steps, noise = [c for c in CASES_FP64 if c[0] == "square_duct_zouhe_c2_rows"][0]
+    cfg = base_cfg(**kw)
+    shape = (cfg.NZ, cfg.NY, cfg.NX)
+    init = (np.ones(shape),) + seeded_velocity(shape, (0, 0, 0.01), noise, seed=7)
+    lat, orc = run_both(cfg, 600, init, prec_gpu=vlbm.FP32)
+    o32 = po.Oracle(po.desc_from(cfg.to_desc()), 4)
+    o32.init(*init)
+    o32.step(600, save_last=True)
+    dr_ref = orc.get_pops()[:, :-1].sum(0) - 1
+    e_gpu = rel_l2(lat.get_pops()[:, :-1].sum(0) - 1, dr_ref)
+    e_ref32 = rel_l2(o32.get_pops()[:, :-1].sum(0) - 1, dr_ref)
+    print(f"rho-1 vs FP64 oracle after 600 steps: FP32 GPU {e_gpu:.2e}, reference FP32 build {e_ref32:.2e}", flush=True)
+    assert e_gpu <= e_ref32 and e_gpu <= 1e-4
+    lat.close()
 
 
 def test_c2_layouts_agree_full_size():

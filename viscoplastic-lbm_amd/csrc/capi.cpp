@@ -324,6 +324,13 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
     *out = nullptr;
     int rc = validate(d);
     if (rc) return rc;
+    // the rheometer's plate velocity divides by the node's distance from the
+    // axis (parallelPlatesRheometer.cu:153-158), which is 0 on the axis node
+    // of an odd x odd cross-section: the reference's state turns NaN there
+    // (its node map is still well defined, so vlbm_build_map accepts it)
+    if (d->builder == VLBM_BUILDER_RHEOMETER && d->nx % 2 == 1 && d->ny % 2 == 1)
+        return fail(VLBM_ERR_INVALID, "the rheometer needs an even NX or NY (odd x odd puts a plate node on the "
+                                      "axis, where the wall velocity is 0/0)");
     vlbm_ctx* h = new (std::nothrow) vlbm_ctx;
     if (!h) return fail(VLBM_ERR_ALLOC, "host allocation failed");
     h->d = *d;
@@ -337,6 +344,7 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
     if (!h->mask_geom) box_region_words(d->builder, h->region_word);
     else for (int r = 0; r < 27; r++) h->region_word[r] = 0;
 
+    if (hipGetDevice(&h->device) != hipSuccess) { release(h); return fail(VLBM_ERR_DEVICE, "hipGetDevice failed"); }
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { release(h); return fail(VLBM_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
@@ -381,6 +389,36 @@ void vlbm_destroy(vlbm_h h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     release(h);
+}
+
+int vlbm_set_map(vlbm_h h, const uint32_t* words) {
+    if (!h || !words) return fail(VLBM_ERR_INVALID, "handle or words NULL");
+    const vlbm_desc& d = h->d;
+    if (d.builder == VLBM_BUILDER_RHEOMETER)
+        return fail(VLBM_ERR_UNSUPPORTED, "the rheometer's map is fixed by its plates");
+    if (h->ibm) return fail(VLBM_ERR_UNSUPPORTED, "set the node map before enabling the immersed boundary stage");
+    for (size_t k = 0; k < h->n; k++) {
+        const uint32_t w = words[k];
+        const int s = w_scheme(w), dir = w_dir(w);
+        if (dir > dSEB) return fail(VLBM_ERR_INVALID, "node " + std::to_string(k) + ": direction code beyond 25");
+        if (s == kSpecial)
+            return fail(VLBM_ERR_UNSUPPORTED, "node " + std::to_string(k) + ": the special scheme is the rheometer's");
+        if ((s == kVelZouHe || s == kPresZouHe) && dir > dB)
+            return fail(VLBM_ERR_INVALID, "node " + std::to_string(k) + ": Zou-He is defined on the six faces only");
+        if (d.q == 27 && (s == kVelZouHe || s == kPresZouHe || s == kVelBB))
+            return fail(VLBM_ERR_UNSUPPORTED, "Zou-He and velocity bounce-back are D3Q19-only");
+        if (s == kInterpBB && !(d.duct_outer_radius > 0))
+            return fail(VLBM_ERR_INVALID, "interpolated bounce-back needs the duct radius");
+    }
+    if (!h->d_mask) {
+        int rc = dalloc(h, (void**)&h->d_mask, h->n * 4);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(h->d_mask, words, h->n * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->words.assign(words, words + h->n);
+    h->mask_geom = true;
+    return VLBM_OK;
 }
 
 int vlbm_bc_nodes(vlbm_h h, uint64_t* out, size_t cap, size_t* count) {
@@ -572,10 +610,11 @@ int vlbm_load_pops(vlbm_h h, const double* pop) {
     int rc = h->mom ? need_pops(h) : VLBM_OK;
     if (rc) return rc;
     if ((rc = upload_ref_pops(h, pop, h->mom ? h->pops : h->lat[0]))) return rc;
-    // the rheometer reads the idle lattice (held values, unused planes) in the
-    // first steps: both lattices take the loaded state, as both buffers do
-    // when the reference loads a checkpoint into pop and popAux
-    if (d.builder == VLBM_BUILDER_RHEOMETER && (rc = upload_ref_pops(h, pop, h->lat[1]))) return rc;
+    // nodes that never collide (outside duct walls, the rheometer's unused
+    // planes) keep the loaded values in both lattices, as in both of the
+    // reference's buffers; the rheometer also reads the idle lattice (held
+    // values) in the first steps
+    if (!h->mom && h->mask_geom && (rc = upload_ref_pops(h, pop, h->lat[1]))) return rc;
     h->cur = 0;
     h->direct = true;
     h->direct_pops = h->mom;
@@ -611,23 +650,38 @@ int vlbm_get_pops(vlbm_h h, double* pop) {
         source = target;
     }
     const size_t total = (size_t)d.q * h->n;
+    const size_t plane = (size_t)d.nx * d.ny;
+    const size_t ghost = (size_t)d.q * plane;
     double* dbl = nullptr;
-    HIP_TRY(hipMalloc((void**)&dbl, total * 8));
-    std::vector<double> host(total);
+    HIP_TRY(hipMalloc((void**)&dbl, (total + ghost) * 8));
+    std::vector<double> host(total + ghost, 0.0);
     int rc = VLBM_OK;
     if (convert_planes(h, (void*)source, dbl, 1)) rc = check_launch(-1, "conversion kernel");
-    else if (hipMemcpyAsync(host.data(), dbl, total * 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-             hipStreamSynchronize(h->stream) != hipSuccess)
+    // the extra plane z = NZ: what the last step pushed across the z ends
+    // (the rest state before any step)
+    if (!rc && !h->direct) {
+        rc = with_params(h, [&](void* pv, auto& p) {
+            p.src = h->lat[h->cur];
+            p.dst = dbl + total;
+            p.aux_mode = kAuxGhost;
+            p.direct = 0;
+            return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
+                                "extra plane kernel");
+        });
+    }
+    if (!rc && (hipMemcpyAsync(host.data(), dbl, (h->direct ? total : total + ghost) * 8, hipMemcpyDeviceToHost,
+                               h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess))
         rc = fail(VLBM_ERR_DEVICE, "population download failed");
     (void)hipFree(dbl);
     if (rc) return rc;
-    const size_t plane = (size_t)d.nx * d.ny;
     for (int i = 0; i < d.q; i++) {
         const double w = pop_weight(d.q, i);
         double* dstp = pop + (size_t)i * plane * (d.nz + 1);
         const double* srcp = host.data() + (size_t)i * h->n;
         for (size_t k = 0; k < h->n; k++) dstp[k] = srcp[k] + w;
-        std::memset(dstp + h->n, 0, plane * sizeof(double));
+        const double* gp = host.data() + total + (size_t)i * plane;
+        for (size_t k = 0; k < plane; k++) dstp[h->n + k] = gp[k] + w;
     }
     return VLBM_OK;
 }

@@ -64,6 +64,7 @@ struct vlbm_ctx {
     double* partials = nullptr;
     double* out3 = nullptr;
     hipStream_t stream = nullptr;
+    int device = 0;           // device current at vlbm_create (the stream's device)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t device_bytes = 0;
     vlbm::IbmState* ibm = nullptr;   // immersed boundary stage (vlbm_ibm_enable)

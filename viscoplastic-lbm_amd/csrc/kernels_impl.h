@@ -132,9 +132,10 @@ __device__ __forceinline__ T load_pop(const Params<T>& p, int i, uint32_t node) 
 //   * velocity BACK uses 1/(1+uz_w) (VelZouHe.cu:153) and u_x in place of u_y
 //     for populations 11 and 18 (VelZouHe.cu:168-171);
 //   * velocity WEST population 15 uses u_y in place of u_z (VelZouHe.cu:95);
-//   * EAST (velocity and pressure) adds to each incoming population its own
-//     streamed value the WEST increment of its opposite (VelZouHe.cu:115-119,
-//     PresZouHe.cu:115-119).
+//   * EAST (velocity and pressure) takes T + 2K from the WEST face's known set
+//     (its own incoming populations) and adds to each incoming population its
+//     own streamed value the WEST increment of its opposite
+//     (VelZouHe.cu:104-119, PresZouHe.cu:103-119).
 // Zou-He works on deviations unchanged: the weights of T + 2K sum to 1 and
 // the tangential sums are antisymmetric.
 template <typename T>
@@ -144,6 +145,10 @@ __device__ __forceinline__ void zou_he(T* g, int dir, bool pressure, T uxw, T uy
     const int a = (dir == dN || dir == dS) ? 1 : ((dir == dW || dir == dE) ? 0 : 2);
     const int s = (dir == dN || dir == dE || dir == dF) ? 1 : -1;
     auto ca = [&](int i) -> int { return a == 0 ? kCX[i] : (a == 1 ? kCY[i] : kCZ[i]); };
+    // the reference's EAST lines sum the WEST face's known set (c_x = -1, the
+    // EAST face's incoming populations) into T + 2K (VelZouHe.cu:104-108,
+    // PresZouHe.cu:103-107)
+    const int known = (compat && dir == dE) ? -s : s;
     T tsum = 0, ksum = 0, n_t[3] = {0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 19; i++) {
@@ -153,7 +158,7 @@ __device__ __forceinline__ void zou_he(T* g, int dir, bool pressure, T uxw, T uy
             n_t[0] += (T)kCX[i] * g[i];
             n_t[1] += (T)kCY[i] * g[i];
             n_t[2] += (T)kCZ[i] * g[i];
-        } else if (c == s) {
+        } else if (c == known) {
             ksum += g[i];
         }
     }
@@ -413,17 +418,49 @@ template <int Q> struct XShifted {
     }
 };
 
-template <int Q, typename S, typename T>
+//
+// BB_IN (box maps): the value a thread loads for population i goes to node
+// x + c_ix of its row. When that node is a bounce-back node that replaces
+// population i by its own post-collision opposite value g*_opp(i), the
+// thread loads that value instead (the pulled one would be discarded), so
+// wall nodes get everything in the first round of loads and the boundary
+// pass skips bounce-back. On the duct's x walls the loads replaced are the
+// ones wrapped around the row.
+template <int Q>
+__device__ __forceinline__ uint32_t bb_bits(uint32_t word) {
+    if (!w_used(word) || w_scheme(word) != kBounceBack) return 0;
+    const int dir = w_dir(word);
+    // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
+    if ((dir >= dNW && w_geom(word)) || dir > dSEB) return 0;
+    return (Q == 19) ? c_bb19.m[dir] : c_bb27.m[dir];
+}
+
+template <int Q, typename S, typename T, bool BB_IN = false>
 __device__ __forceinline__ void gather_rows(T* g, const Params<T>& p, const Node<T>& nd, T* sm) {
     const int t = (int)threadIdx.x;
     const uint32_t row = row_off(p, nd.y, nd.z) - blockIdx.x * (uint32_t)kBlock;
     const uint32_t t_left = row + (uint32_t)nd.xm, t_right = row + (uint32_t)nd.xp;   // threads of x -+ 1
     const bool left_in = t_left < (uint32_t)kBlock, right_in = t_right < (uint32_t)kBlock;
+    uint32_t bb_m = 0, bb_0 = 0, bb_p = 0;     // bounce-back masks of the nodes x - 1, x, x + 1
+    if constexpr (BB_IN) {
+        bb_m = bb_bits<Q>(p.region_word[box_region(nd.xm, nd.y, nd.z, p.nx, p.ny, p.nz)]);
+        bb_0 = bb_bits<Q>(p.region_word[box_region(nd.x, nd.y, nd.z, p.nx, p.ny, p.nz)]);
+        bb_p = bb_bits<Q>(p.region_word[box_region(nd.xp, nd.y, nd.z, p.nx, p.ny, p.nz)]);
+    }
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         const int ys = kCY[i] > 0 ? nd.ym : (kCY[i] < 0 ? nd.yp : nd.y);
         const int zs = kCZ[i] > 0 ? nd.zm : (kCZ[i] < 0 ? nd.zp : nd.z);
-        g[i] = ld_nt<S, T>((const S*)p.src + (int64_t)i * p.stride + row_off(p, ys, zs) + nd.x);
+        int64_t off = (int64_t)i * p.stride + row_off(p, ys, zs) + nd.x;
+        if constexpr (BB_IN) {
+            if (i > 0) {
+                const uint32_t bits = kCX[i] > 0 ? bb_p : (kCX[i] < 0 ? bb_m : bb_0);
+                if ((bits >> i) & 1u)
+                    off = (int64_t)opp(i) * p.stride + row_off(p, nd.y, nd.z) +
+                          (kCX[i] > 0 ? nd.xp : (kCX[i] < 0 ? nd.xm : nd.x));
+            }
+        }
+        g[i] = ld_nt<S, T>((const S*)p.src + off);
     }
 #pragma unroll
     for (int i = 0; i < Q; i++)
@@ -506,7 +543,9 @@ __device__ __forceinline__ void plate_held(T* g, const Params<T>& p, const Node<
 // come from a per-node map (duct and rheometer builders). Box builders' words
 // never carry interpolated bounce-back or the rheometer's special scheme, so
 // their kernels are compiled without that code.
-template <int Q, typename T, bool MASK, class PC>
+// BB_DONE: the gather already delivered the bounce-back values (gather_rows
+// with BB_IN).
+template <int Q, typename T, bool MASK, bool BB_DONE = false, class PC>
 __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>& nd, uint32_t word, PC pc) {
     const int scheme = w_scheme(word);
     const int dir = w_dir(word);
@@ -514,13 +553,15 @@ __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>
         if (p.plates) plate_held<Q, T>(g, p, nd, pc);
     }
     if (scheme == kBounceBack) {
-        // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
-        if (dir >= dNW && w_geom(word)) return;
-        if (dir > dSEB) return;
-        const uint32_t m = (Q == 19) ? c_bb19.m[dir] : c_bb27.m[dir];
+        if constexpr (!BB_DONE) {
+            // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
+            if (dir >= dNW && w_geom(word)) return;
+            if (dir > dSEB) return;
+            const uint32_t m = (Q == 19) ? c_bb19.m[dir] : c_bb27.m[dir];
 #pragma unroll
-        for (int i = 1; i < Q; i++)
-            if (m & (1u << i)) g[i] = pc(opp(i), 0, 0, 0);
+            for (int i = 1; i < Q; i++)
+                if (m & (1u << i)) g[i] = pc(opp(i), 0, 0, 0);
+        }
     } else if (scheme == kFreeSlip) {
         // specular reflection: incoming i comes from the mirror population m
         // of the node displaced by minus the tangential part of c_i
@@ -810,9 +851,9 @@ __device__ __forceinline__ void step_node_rows(const Params<typename ComputeOf<S
     const Node<T> nd = make_node(p, idx);
     const uint32_t word = node_word<MASK>(p, nd);
     T g[Q];
-    gather_rows<Q, S, T>(g, p, nd, sm);
+    gather_rows<Q, S, T, !MASK>(g, p, nd, sm);
     if (!in || !w_used(word)) return;
-    if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, GlobalPC<Q, S, T, kRepPop>{p, nd});
+    if (w_scheme(word) != kNull) apply_bc<Q, T, MASK, !MASK>(g, p, nd, word, GlobalPC<Q, S, T, kRepPop>{p, nd});
     collide_store<Q, S, RHEO, IBM, OUT>(p, idx, nd, g);
 }
 
@@ -837,11 +878,41 @@ __global__ __launch_bounds__(kBlock) void k_step(const Params<typename ComputeOf
 // advancing it:
 //   kAuxMoments:     macroscopics the next step would compute
 //   kAuxMaterialize: post-stream, post-BC populations into dst (Q planes)
+//   kAuxGhost:       the reference's extra z plane after a step (doubles)
+// Nodes that do not collide (outside a duct wall, the rheometer's planes
+// z = 0 and NZ-1) hold what the reference's push streaming leaves in them:
+// the post-collision value of a colliding source node x - c_i, the extra
+// plane's rest state where the stream wraps in z from a non-colliding node,
+// and otherwise their initial (or loaded) value.
+// The pass covers nodes [aux_base, aux_base + aux_count) (every node when
+// aux_count is 0); kAuxGhost and the double output of kAuxMaterialize
+// (aux_f64) write planes of aux_count values.
 template <int Q, typename S, int RHEO, bool MASK, int IN = kRepPop>
 __global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<S>::type> p) {
     using T = typename ComputeOf<S>::type;
-    const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
-    if (idx >= p.n) return;
+    const uint32_t plane = (uint32_t)p.nx * (uint32_t)p.ny;
+    const uint32_t count = p.aux_count ? p.aux_count : (p.aux_mode == kAuxGhost ? plane : p.n);
+    const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
+    if (local >= count) return;
+    const uint32_t idx = p.aux_base + local;
+    if (p.aux_mode == kAuxGhost) {
+        // plane z = NZ of the reference: the populations pushed across the z
+        // ends before the fold (c_z > 0 from z = NZ-1, c_z < 0 from z = 0),
+        // the rest state w_i where the source does not collide or c_z = 0
+        double* out = (double*)p.dst;
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            double v = 0;
+            if (kCZ[i] != 0) {
+                const Node<T> nd = make_node(p, idx + (kCZ[i] > 0 ? (uint32_t)(p.nz - 1) * plane : 0u));
+                const GlobalPC<Q, S, T, IN> pc{p, nd};
+                const Node<T> src = make_node(p, node_at(p, nd, -kCX[i], -kCY[i], 0));
+                if (w_used(node_word<MASK>(p, src))) v = (double)pc(i, -kCX[i], -kCY[i], 0);
+            }
+            out[(int64_t)i * count + local] = v;
+        }
+        return;
+    }
     const Node<T> nd = make_node(p, idx);
     if (p.aux_mode == kAuxMomentsOutsideTiles && p.tile_flag[tile_of(p, nd.x, nd.y, nd.z)]) return;
     const uint32_t word = node_word<MASK>(p, nd);
@@ -849,14 +920,29 @@ __global__ __launch_bounds__(kBlock) void k_aux(const Params<typename ComputeOf<
     T g[Q];
     if (p.direct || !w_used(word)) {
         load_direct<Q, T>(g, pc);
+        if (MASK && !p.direct && p.aux_mode == kAuxMaterialize) {
+#pragma unroll
+            for (int i = 0; i < Q; i++) {
+                const Node<T> src = make_node(p, node_at(p, nd, -kCX[i], -kCY[i], -kCZ[i]));
+                if (w_used(node_word<MASK>(p, src))) g[i] = pc(i, -kCX[i], -kCY[i], -kCZ[i]);
+                else if ((nd.z == 0 && kCZ[i] > 0) || (nd.z == p.nz - 1 && kCZ[i] < 0)) g[i] = (T)0;
+            }
+        }
     } else {
         gather<Q, T>(g, pc);
         if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, pc);
     }
     if (p.aux_mode == kAuxMaterialize) {
-        S* dst = (S*)p.dst;
+        if (p.aux_f64) {
+            // rounded to the storage type first, as the population planes hold them
+            double* out = (double*)p.dst;
 #pragma unroll
-        for (int i = 0; i < Q; i++) dst[(int64_t)i * p.stride + idx] = (S)g[i];
+            for (int i = 0; i < Q; i++) out[(int64_t)i * count + local] = (double)(T)(S)g[i];
+        } else {
+            S* dst = (S*)p.dst;
+#pragma unroll
+            for (int i = 0; i < Q; i++) dst[(int64_t)i * p.stride + idx] = (S)g[i];
+        }
         return;
     }
     if (!w_used(word)) return;

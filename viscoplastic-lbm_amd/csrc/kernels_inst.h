@@ -81,7 +81,7 @@ struct Inst {
     template <int Q, int RHEO>
     static int aux_q(bool mask, int in_rep, const void* pv, hipStream_t st) {
         const Params<T>& p = *(const Params<T>*)pv;
-        const dim3 grid = grid_of(p.n);
+        const dim3 grid = grid_of(p.aux_count ? p.aux_count : p.n);
         if (in_rep == kRepMom) {
             if constexpr (kHasMom) {
                 if (mask) hipLaunchKernelGGL((k_aux<Q, S, RHEO, true, kRepMom>), grid, dim3(kBlock), 0, st, p);

@@ -144,8 +144,10 @@ struct Params {
     int zouhe_textbook;
     int bgk;                 // BGK collision instead of the regularized one
     T tau, les_c;            // LES: tau_t = (sqrt(tau^2 + les_c |S|) - tau) / 2 when les_c != 0
-    int aux_mode;               // aux kernel: kAuxMoments / kAuxMaterialize
+    int aux_mode;               // aux kernel: AuxMode
     int direct;                 // aux kernel: state is post-stream (no gather, no BC)
+    uint32_t aux_base, aux_count;   // aux kernel: node range (count 0: every node, or every node of the extra plane)
+    int aux_f64;                // kAuxMaterialize: doubles into dst, planes of aux_count values
     T duct_R, duct_r, w_out, u_out, w_in, u_in;
     // immersed boundary: nodes in 8^3 tiles flagged by the last IBM stage take
     // their force from ibm_f*, every other node the constant (fx, fy, fz)
@@ -163,7 +165,9 @@ constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
 
 // kAuxMomentsOutsideTiles: moments with the constant force for nodes outside
 // the flagged tiles; flagged tiles keep the macroscopics of the IBM stage.
-enum AuxMode : int { kAuxMoments = 0, kAuxMaterialize = 1, kAuxMomentsOutsideTiles = 2 };
+// kAuxGhost: the extra z plane of the reference's population layout (deviations
+// as doubles, Q planes of NX*NY, into dst).
+enum AuxMode : int { kAuxMoments = 0, kAuxMaterialize = 1, kAuxMomentsOutsideTiles = 2, kAuxGhost = 3 };
 
 // State representations: Q planes of post-collision deviations, or the 10
 // coefficient planes of the regularized post-collision state.

@@ -97,6 +97,14 @@ class Lattice:
         check(lib.vlbm_get_pops(self._h, _ptr(out)))
         return out
 
+    def set_map(self, words: np.ndarray) -> None:
+        """Replaces the node map with words written outside the library
+        ([NZ][NY][NX] uint32 in the layout of build_map)."""
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        if w.size != self.shape[0] * self.shape[1] * self.shape[2]:
+            raise ValueError("the node map must have NX*NY*NZ words")
+        check(lib.vlbm_set_map(self._h, w.ctypes.data_as(C.POINTER(C.c_uint32))))
+
     def set_omega(self, omega: np.ndarray) -> None:
         check(lib.vlbm_set_omega(self._h, _ptr(_f64(omega))))
 
