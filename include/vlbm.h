@@ -148,6 +148,16 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
 int vlbm_load_pops(vlbm_h h, const double* pop);
 int vlbm_get_pops(vlbm_h h, double* pop);
 
+/* vlbm_get_pops without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops snapshotting states between steps, harnesses that forbid hipMalloc).
+ * Same output as vlbm_get_pops. The values pass, chunk by chunk, through one
+ * 8 MiB device array per GPU, held from the first call on that GPU until the
+ * process ends; lattices of any size are served in chunks of 55,040 (D3Q19)
+ * or 38,656 (D3Q27) nodes. Calls that overlap on one GPU share that array and
+ * race; the calls of one thread are ordered. */
+int vlbm_get_pops_noalloc(vlbm_h h, double* pop);
+
 /* Power-law apparent omega state, NX*NY*NZ doubles (macr.omega). */
 int vlbm_set_omega(vlbm_h h, const double* omega);
 

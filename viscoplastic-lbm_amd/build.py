@@ -40,6 +40,7 @@ SOURCES = [
     "batch_capi.cpp",
     "checkpoint.cpp",
     "capi.cpp",
+    "noalloc.hip",
 ]
 
 

@@ -91,6 +91,7 @@ def _load() -> C.CDLL:
         "vlbm_init": (C.c_int, [C.c_void_p, dp, dp, dp, dp]),
         "vlbm_load_pops": (C.c_int, [C.c_void_p, dp]),
         "vlbm_get_pops": (C.c_int, [C.c_void_p, dp]),
+        "vlbm_get_pops_noalloc": (C.c_int, [C.c_void_p, dp]),
         "vlbm_set_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
         "vlbm_set_omega": (C.c_int, [C.c_void_p, dp]),
         "vlbm_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
