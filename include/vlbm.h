@@ -31,6 +31,13 @@ extern "C" {
 #define VLBM_FP64 8
 #define VLBM_FP32 4
 #define VLBM_FP16 2             /* FP16 deviations f_i - w_i, FP32 arithmetic */
+#define VLBM_BF16 3             /* bfloat16 deviations f_i - w_i, FP32 arithmetic (2 bytes) */
+
+/* rounding of 16-bit storage: stochastic (unbiased: the stored value is the
+ * computed one in expectation, so increments below half a spacing survive on
+ * average) or to nearest */
+#define VLBM_ROUND_STOCHASTIC 0
+#define VLBM_ROUND_NEAREST 1   /* batched FP16 jobs (vlbm_batch_*) always round to nearest */
 
 /* rheology (src/CUDA/NNF/nnf.h:10-30) */
 #define VLBM_NEWTONIAN 0
@@ -81,7 +88,7 @@ extern "C" {
 typedef struct vlbm_desc {
     int nx, ny, nz;          /* NX, NY, NZ (var.h:97-101); one GPU holds the whole lattice */
     int q;                   /* 19 or 27 (var.h D3Q19 / D3Q27) */
-    int precision;           /* VLBM_FP64, VLBM_FP32 or VLBM_FP16 */
+    int precision;           /* VLBM_FP64, VLBM_FP32, VLBM_FP16 or VLBM_BF16 */
     int rheology;            /* VLBM_NEWTONIAN / VLBM_BINGHAM / VLBM_POWERLAW */
     int builder;             /* VLBM_BUILDER_* */
     double tau;              /* TAU: Newtonian relaxation time (OMEGA = 1/TAU) */
@@ -98,6 +105,7 @@ typedef struct vlbm_desc {
     int collision;           /* VLBM_COLLISION_REGULARIZED or VLBM_COLLISION_BGK */
     double les_const;        /* LES_MODEL's Implicit_const (lbm.cu:226-242); 0 = no LES, Newtonian only */
     int layout;              /* VLBM_LAYOUT_* (not in the reference: its state is always populations) */
+    int rounding;            /* VLBM_ROUND_*: rounding of 16-bit storage (not in the reference) */
 } vlbm_desc;
 
 typedef struct vlbm_ctx* vlbm_h;

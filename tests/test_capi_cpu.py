@@ -91,6 +91,20 @@ def test_rheometer_refusals(kw, code):
     assert e.value.code == code and "rheometer" in str(e.value)
 
 
+def test_yaml_surface_of_the_reference_example():
+    """The reference's own example (examples/all_options.yaml, committed as
+    tests/golden/all_options.yaml) reads into the descriptor pyvisc would
+    render into var.h: ID 001, FP32, 32^3 D3Q19 at tau 0.9, no force,
+    Newtonian, the default builder, the report cadence and residual."""
+    from pathlib import Path
+    cfg = vlbm.read_configs(str(Path(__file__).resolve().parent / "golden" / "all_options.yaml"))
+    assert cfg.id_sim == "001" and cfg.steps == 10000 and cfg.macr_save == 1000 and cfg.data_report == 1000
+    assert cfg.resid_max == 1e-5 and cfg.path_save == "results" and not cfg.data_stop and cfg.ini_step == 0
+    d = cfg.to_desc()
+    assert (d.nx, d.ny, d.nz, d.q, d.precision, d.rheology) == (32, 32, 32, 19, 4, 0)
+    assert (d.fx, d.fy, d.fz) == (0.0, 0.0, 0.0) and np.isclose(d.tau, 0.9)
+
+
 def test_yaml_surface(tmp_path):
     text = """
 - sim_id: 1

@@ -33,6 +33,7 @@ SOURCES = [
     "kernels_fp64.hip",
     "kernels_fp32.hip",
     "kernels_fp16.hip",
+    "kernels_bf16.hip",
     "kernels_common.hip",
     "kernels_ibm.hip",
     "ibm_capi.cpp",

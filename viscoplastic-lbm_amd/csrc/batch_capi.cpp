@@ -145,6 +145,7 @@ Layout layout_of(const vlbm_desc& d, int tbytes) {
 template <typename T>
 void job_params(vlbm_batch* b, const Job& j, int src, Params<T>& p) {
     fill_params_desc(j.d, j.words, p);
+    p.sr = 0;   // the captured step sequences replay fixed parameters: FP16 jobs round to nearest
     const Layout L = layout_of(j.d, b->tbytes);
     char* base = b->arena + j.off;
     char* lat[2] = {base, base + L.lat};

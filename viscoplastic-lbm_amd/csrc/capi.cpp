@@ -203,8 +203,11 @@ int validate(const vlbm_desc* d) {
     const double n = (double)d->nx * d->ny * d->nz;
     if (n >= 2147483648.0) return fail(VLBM_ERR_INVALID, "lattice has 2^31 nodes or more");
     if (d->q != 19 && d->q != 27) return fail(VLBM_ERR_INVALID, "velocity set must be D3Q19 or D3Q27");
-    if (d->precision != VLBM_FP64 && d->precision != VLBM_FP32 && d->precision != VLBM_FP16)
-        return fail(VLBM_ERR_INVALID, "precision must be 8, 4 or 2 bytes");
+    if (d->precision != VLBM_FP64 && d->precision != VLBM_FP32 && d->precision != VLBM_FP16 &&
+        d->precision != VLBM_BF16)
+        return fail(VLBM_ERR_INVALID, "precision must be VLBM_FP64, VLBM_FP32, VLBM_FP16 or VLBM_BF16");
+    if (d->rounding != VLBM_ROUND_STOCHASTIC && d->rounding != VLBM_ROUND_NEAREST)
+        return fail(VLBM_ERR_INVALID, "rounding must be VLBM_ROUND_STOCHASTIC or VLBM_ROUND_NEAREST");
     if (d->rheology < 0 || d->rheology > 2) return fail(VLBM_ERR_INVALID, "unknown rheology");
     if (!is_box_builder(d->builder) && !is_duct_builder(d->builder))
         return fail(VLBM_ERR_INVALID, "unknown boundary-condition builder");
@@ -220,7 +223,8 @@ int validate(const vlbm_desc* d) {
         return fail(VLBM_ERR_UNSUPPORTED, "LES applies to Newtonian fluids only");
     if (d->les_const > 0 && !(d->tau > 0.5)) return fail(VLBM_ERR_INVALID, "LES needs tau > 0.5");
     if (d->layout < VLBM_LAYOUT_AUTO || d->layout > VLBM_LAYOUT_MOMENTS) return fail(VLBM_ERR_INVALID, "unknown layout");
-    if (d->layout == VLBM_LAYOUT_MOMENTS && (d->collision != VLBM_COLLISION_REGULARIZED || d->precision == VLBM_FP16))
+    if (d->layout == VLBM_LAYOUT_MOMENTS && (d->collision != VLBM_COLLISION_REGULARIZED || d->precision == VLBM_FP16 ||
+                                             d->precision == VLBM_BF16))
         return fail(VLBM_ERR_UNSUPPORTED, "the moment layout holds the regularized state in float or double storage");
     if (d->q == 27 && d->builder != VLBM_BUILDER_TAYLOR_GREEN) {
         // Zou-He and velocity bounce-back exist for D3Q19 only in the reference
@@ -336,7 +340,7 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
     h->d = *d;
     h->n = (size_t)d->nx * d->ny * d->nz;
     h->stride = plane_stride(h->n);
-    h->sbytes = d->precision;
+    h->sbytes = d->precision == VLBM_BF16 ? 2 : d->precision;
     h->tbytes = d->precision == VLBM_FP64 ? 8 : 4;
     rc = build_words(*d, h->words);
     if (rc) { release(h); return rc; }
@@ -463,7 +467,7 @@ int vlbm_init(vlbm_h h, const double* rho, const double* ux, const double* uy, c
         // planes) hold the initial state, as both of the reference's buffers do
         bool failed = false;
         for (int k = 0; k < 2 && !failed; k++)
-            failed = launch_fill_feq(h->d.q, h->sbytes, h->lat[k], h->stride, (uint32_t)n, dev, dev + n, dev + 2 * n,
+            failed = launch_fill_feq(h->d.q, h->d.precision, h->lat[k], h->stride, (uint32_t)n, dev, dev + n, dev + 2 * n,
                                      dev + 3 * n, h->stream, h->mom ? kRepMom : kRepPop) != 0;
         if (failed) {
             rc = check_launch(-1, "equilibrium kernel");
@@ -510,7 +514,7 @@ int vlbm_set_omega(vlbm_h h, const double* omega) {
 // planes of n doubles.
 static int convert_planes(vlbm_h h, void* state, double* dbl, int to_double) {
     for (int i = 0; i < h->d.q; i++)
-        if (launch_convert(h->sbytes, (char*)state + (size_t)i * (size_t)h->stride * h->sbytes, dbl + (size_t)i * h->n,
+        if (launch_convert(h->d.precision, (char*)state + (size_t)i * (size_t)h->stride * h->sbytes, dbl + (size_t)i * h->n,
                            h->n, to_double, h->stream))
             return -1;
     return 0;
@@ -643,7 +647,7 @@ int vlbm_get_pops(vlbm_h h, double* pop) {
             p.dst = target;
             p.aux_mode = kAuxMaterialize;
             p.direct = h->direct ? 1 : 0;
-            return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
+            return check_launch(launch_aux(d.q, d.precision, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
                                 "materialize kernel");
         });
         if (rc) return rc;
@@ -665,7 +669,7 @@ int vlbm_get_pops(vlbm_h h, double* pop) {
             p.dst = dbl + total;
             p.aux_mode = kAuxGhost;
             p.direct = 0;
-            return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
+            return check_launch(launch_aux(d.q, d.precision, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
                                 "extra plane kernel");
         });
     }
@@ -701,7 +705,8 @@ static int enqueue_steps(vlbm_h h, int n_steps, int save_last, hipEvent_t* ev = 
             // IBM builds take the macroscopics from the stage (save_macr_to_array
             // is false with IBM, src/CUDA/main.cu:293-297)
             p.save = (save && !ibm) ? 1 : 0;
-            return check_launch(launch_step(d.q, h->sbytes, d.rheology, h->mask_geom, h->direct, pv, h->stream, ibm,
+            p.sr_seed = (uint32_t)h->steps * 0x9e3779b9u + 0x7f4a7c15u;
+            return check_launch(launch_step(d.q, d.precision, d.rheology, h->mask_geom, h->direct, pv, h->stream, ibm,
                                             state_rep(h), h->mom ? kRepMom : kRepPop),
                                 "step kernel");
         });
@@ -787,7 +792,7 @@ int vlbm_compute_macroscopics(vlbm_h h) {
         p.dst = h->lat[1 - h->cur];
         p.aux_mode = kAuxMoments;
         p.direct = h->direct ? 1 : 0;
-        return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
+        return check_launch(launch_aux(d.q, d.precision, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
                             "moments kernel");
     });
 }

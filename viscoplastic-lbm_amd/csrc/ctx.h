@@ -38,7 +38,7 @@ struct vlbm_ctx {
     vlbm_desc d;
     size_t n = 0;             // nodes
     int64_t stride = 0;       // elements between population planes
-    int sbytes = 0;           // storage bytes per population
+    int sbytes = 0;           // storage bytes per population (the launchers take d.precision)
     int tbytes = 0;           // compute / macroscopic bytes per value
     void* lat[2] = {nullptr, nullptr};
     int cur = 0;              // lattice holding the current state
@@ -111,6 +111,7 @@ inline void fill_params_desc(const vlbm_desc& d, const uint32_t region_word[27],
     p.w_out = (T)d.outer_rotation; p.u_out = (T)d.outer_velocity;
     p.w_in = (T)d.inner_rotation; p.u_in = (T)d.inner_velocity;
     p.plates = d.builder == VLBM_BUILDER_RHEOMETER ? 1 : 0;
+    p.sr = (d.precision == VLBM_FP16 || d.precision == VLBM_BF16) && d.rounding == VLBM_ROUND_STOCHASTIC ? 1 : 0;
 }
 
 template <typename T>

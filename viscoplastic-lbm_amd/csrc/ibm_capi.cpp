@@ -280,7 +280,7 @@ int ibm_stage(vlbm_ctx* h) {
         p.dst = h->lat[1 - h->cur];
         p.direct = h->direct ? 1 : 0;
         p.ibm_stage = (StageRec<T>*)s.stage;
-        return check_launch(launch_ibm_macr(d.q, h->sbytes, h->mask_geom, pv, nullptr, s.P.tile_list, s.P.tile_count,
+        return check_launch(launch_ibm_macr(d.q, d.precision, h->mask_geom, pv, nullptr, s.P.tile_list, s.P.tile_count,
                                             max_tiles, h->stream, state_rep(h)),
                             "IBM macroscopics kernel");
     });
@@ -300,7 +300,7 @@ int ibm_fill_macr(vlbm_ctx* h) {
         p.dst = h->lat[1 - h->cur];
         p.aux_mode = kAuxMomentsOutsideTiles;
         p.direct = h->direct ? 1 : 0;
-        return check_launch(launch_aux(d.q, h->sbytes, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
+        return check_launch(launch_aux(d.q, d.precision, d.rheology, h->mask_geom, pv, h->stream, state_rep(h)),
                             "moments kernel");
     });
     if (rc) return rc;

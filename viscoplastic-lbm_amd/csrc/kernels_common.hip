@@ -19,6 +19,11 @@ int launch_ibm_macr_fp16(int q, bool mask, int in_rep, const void* p, const uint
 int launch_aux_fp64(int q, int rheo, bool mask, int in_rep, const void* p, void* stream);
 int launch_aux_fp32(int q, int rheo, bool mask, int in_rep, const void* p, void* stream);
 int launch_aux_fp16(int q, int rheo, bool mask, int in_rep, const void* p, void* stream);
+int launch_step_bf16(int q, int rheo, bool mask, bool direct, bool ibm, int in_rep, int out_rep, const void* p,
+                     void* stream);
+int launch_ibm_macr_bf16(int q, bool mask, int in_rep, const void* p, const uint32_t* list, const uint32_t* count,
+                         uint32_t max_tiles, void* stream);
+int launch_aux_bf16(int q, int rheo, bool mask, int in_rep, const void* p, void* stream);
 
 __global__ void k_reduce_final(const double* partials, int nblocks, double* out3) {
     // one wave; fixed order -> deterministic
@@ -41,6 +46,7 @@ int launch_step(int q, int prec, int rheo, bool mask, bool direct, const void* p
     if (prec == 8) return launch_step_fp64(q, rheo, mask, direct, ibm, in_rep, out_rep, p, stream);
     if (prec == 4) return launch_step_fp32(q, rheo, mask, direct, ibm, in_rep, out_rep, p, stream);
     if (in_rep != kRepPop || out_rep != kRepPop) return -1;
+    if (prec == 3) return launch_step_bf16(q, rheo, mask, direct, ibm, in_rep, out_rep, p, stream);
     return launch_step_fp16(q, rheo, mask, direct, ibm, in_rep, out_rep, p, stream);
 }
 
@@ -48,12 +54,14 @@ int launch_ibm_macr(int q, int prec, bool mask, const void* p, const void* /*flu
                     const uint32_t* count, uint32_t max_tiles, void* stream, int in_rep) {
     if (prec == 8) return launch_ibm_macr_fp64(q, mask, in_rep, p, list, count, max_tiles, stream);
     if (prec == 4) return launch_ibm_macr_fp32(q, mask, in_rep, p, list, count, max_tiles, stream);
+    if (prec == 3) return launch_ibm_macr_bf16(q, mask, in_rep, p, list, count, max_tiles, stream);
     return launch_ibm_macr_fp16(q, mask, in_rep, p, list, count, max_tiles, stream);
 }
 
 int launch_aux(int q, int prec, int rheo, bool mask, const void* p, void* stream, int in_rep) {
     if (prec == 8) return launch_aux_fp64(q, rheo, mask, in_rep, p, stream);
     if (prec == 4) return launch_aux_fp32(q, rheo, mask, in_rep, p, stream);
+    if (prec == 3) return launch_aux_bf16(q, rheo, mask, in_rep, p, stream);
     return launch_aux_fp16(q, rheo, mask, in_rep, p, stream);
 }
 
@@ -67,9 +75,11 @@ int launch_fill_feq(int q, int prec, void* state, int64_t stride, uint32_t n, co
         else if (prec == 4) VLBM_FILL(19, float, kRepMom);
         else return -1;
     } else if (q == 19) {
-        if (prec == 8) VLBM_FILL(19, double, kRepPop); else if (prec == 4) VLBM_FILL(19, float, kRepPop); else VLBM_FILL(19, _Float16, kRepPop);
+        if (prec == 8) VLBM_FILL(19, double, kRepPop); else if (prec == 4) VLBM_FILL(19, float, kRepPop);
+        else if (prec == 3) VLBM_FILL(19, bf16, kRepPop); else VLBM_FILL(19, _Float16, kRepPop);
     } else {
-        if (prec == 8) VLBM_FILL(27, double, kRepPop); else if (prec == 4) VLBM_FILL(27, float, kRepPop); else VLBM_FILL(27, _Float16, kRepPop);
+        if (prec == 8) VLBM_FILL(27, double, kRepPop); else if (prec == 4) VLBM_FILL(27, float, kRepPop);
+        else if (prec == 3) VLBM_FILL(27, bf16, kRepPop); else VLBM_FILL(27, _Float16, kRepPop);
     }
 #undef VLBM_FILL
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -110,6 +120,7 @@ int launch_convert(int prec, void* buf, double* dbl, size_t count, int to_double
     const dim3 grid((unsigned)((count + kBlock - 1) / kBlock));
     if (prec == 8) hipLaunchKernelGGL((k_convert<double>), grid, dim3(kBlock), 0, st, (double*)buf, dbl, count, to_double);
     else if (prec == 4) hipLaunchKernelGGL((k_convert<float>), grid, dim3(kBlock), 0, st, (float*)buf, dbl, count, to_double);
+    else if (prec == 3) hipLaunchKernelGGL((k_convert<bf16>), grid, dim3(kBlock), 0, st, (bf16*)buf, dbl, count, to_double);
     else hipLaunchKernelGGL((k_convert<_Float16>), grid, dim3(kBlock), 0, st, (_Float16*)buf, dbl, count, to_double);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

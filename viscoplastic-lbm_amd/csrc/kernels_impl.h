@@ -37,14 +37,74 @@ static __constant__ BBTable<19> c_bb19 = make_bb_table<19>();
 static __constant__ BBTable<27> c_bb27 = make_bb_table<27>();
 
 // ---- storage conversions ------------------------------------------------------
+// bfloat16 storage: the upper 16 bits of a float (8 exponent bits, 7 stored
+// mantissa bits); conversion from float rounds to nearest even.
+struct bf16 {
+    uint16_t bits;
+    bf16() = default;
+    __device__ explicit bf16(float f) {
+        uint32_t u = __float_as_uint(f);
+        if ((u & 0x7f800000u) == 0x7f800000u) {
+            bits = (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));   // inf stays inf, NaN stays NaN
+        } else {
+            u += 0x7fffu + ((u >> 16) & 1u);
+            bits = (uint16_t)(u >> 16);
+        }
+    }
+    __device__ explicit bf16(double d) : bf16((float)d) {}
+    __device__ operator float() const { return __uint_as_float((uint32_t)bits << 16); }
+};
+
 template <typename S, typename T> __device__ __forceinline__ T ld(const S* p) { return (T)(*p); }
 template <> __device__ __forceinline__ float ld<_Float16, float>(const _Float16* p) { return (float)(*p); }
 template <typename S, typename T> __device__ __forceinline__ T ld_nt(const S* p) {
     return (T)__builtin_nontemporal_load(p);
 }
+template <> __device__ __forceinline__ float ld_nt<bf16, float>(const bf16* p) {
+    return __uint_as_float((uint32_t)__builtin_nontemporal_load((const uint16_t*)p) << 16);
+}
 
 template <typename S, typename T> __device__ __forceinline__ void st_nt(S* p, T v) {
     __builtin_nontemporal_store((S)v, p);
+}
+template <> __device__ __forceinline__ void st_nt<bf16, float>(bf16* p, float v) {
+    __builtin_nontemporal_store(bf16(v).bits, (uint16_t*)p);
+}
+
+// Stochastic rounding to 16-bit storage: v is rounded up in magnitude with
+// probability equal to its distance past the representable value below it,
+// in units of the spacing, so the stored value is v in expectation. The
+// per-step forcing of a slow flow is smaller than half an FP16/BF16 spacing
+// of the populations; rounding to nearest drops it every step, stochastic
+// rounding keeps it on average. rnd: uniform random bits (the low 13 for
+// FP16, 16 for BF16). Adding them below the kept bits of the float and
+// truncating rounds the magnitude up exactly with that probability.
+template <typename S>
+__device__ __forceinline__ void st_nt_sr(S* p, float v, uint32_t rnd);
+template <>
+__device__ __forceinline__ void st_nt_sr<_Float16>(_Float16* p, float v, uint32_t rnd) {
+    const float a = fabsf(v);
+    // normal FP16 range: 13 float mantissa bits below the half's last bit;
+    // below it (subnormal halves, spacing 6e-8) and above, to nearest
+    if (a >= 6.103515625e-05f && a < 65504.0f) v = __uint_as_float((__float_as_uint(v) + (rnd & 0x1fffu)) & ~0x1fffu);
+    __builtin_nontemporal_store((_Float16)v, p);
+}
+template <>
+__device__ __forceinline__ void st_nt_sr<bf16>(bf16* p, float v, uint32_t rnd) {
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7f800000u) != 0x7f800000u) u += rnd & 0xffffu;   // inf and NaN as they are
+    __builtin_nontemporal_store((uint16_t)(u >> 16), (uint16_t*)p);
+}
+
+// 32-bit integer mixer (the "lowbias32" finalizer found by C. Wellons' hash
+// prospector): consecutive inputs give independent-looking outputs.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
 }
 
 constexpr int kBlock = 256;      // threads (nodes) per block of the node kernels
@@ -418,49 +478,17 @@ template <int Q> struct XShifted {
     }
 };
 
-//
-// BB_IN (box maps): the value a thread loads for population i goes to node
-// x + c_ix of its row. When that node is a bounce-back node that replaces
-// population i by its own post-collision opposite value g*_opp(i), the
-// thread loads that value instead (the pulled one would be discarded), so
-// wall nodes get everything in the first round of loads and the boundary
-// pass skips bounce-back. On the duct's x walls the loads replaced are the
-// ones wrapped around the row.
-template <int Q>
-__device__ __forceinline__ uint32_t bb_bits(uint32_t word) {
-    if (!w_used(word) || w_scheme(word) != kBounceBack) return 0;
-    const int dir = w_dir(word);
-    // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
-    if ((dir >= dNW && w_geom(word)) || dir > dSEB) return 0;
-    return (Q == 19) ? c_bb19.m[dir] : c_bb27.m[dir];
-}
-
-template <int Q, typename S, typename T, bool BB_IN = false>
+template <int Q, typename S, typename T>
 __device__ __forceinline__ void gather_rows(T* g, const Params<T>& p, const Node<T>& nd, T* sm) {
     const int t = (int)threadIdx.x;
     const uint32_t row = row_off(p, nd.y, nd.z) - blockIdx.x * (uint32_t)kBlock;
     const uint32_t t_left = row + (uint32_t)nd.xm, t_right = row + (uint32_t)nd.xp;   // threads of x -+ 1
     const bool left_in = t_left < (uint32_t)kBlock, right_in = t_right < (uint32_t)kBlock;
-    uint32_t bb_m = 0, bb_0 = 0, bb_p = 0;     // bounce-back masks of the nodes x - 1, x, x + 1
-    if constexpr (BB_IN) {
-        bb_m = bb_bits<Q>(p.region_word[box_region(nd.xm, nd.y, nd.z, p.nx, p.ny, p.nz)]);
-        bb_0 = bb_bits<Q>(p.region_word[box_region(nd.x, nd.y, nd.z, p.nx, p.ny, p.nz)]);
-        bb_p = bb_bits<Q>(p.region_word[box_region(nd.xp, nd.y, nd.z, p.nx, p.ny, p.nz)]);
-    }
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         const int ys = kCY[i] > 0 ? nd.ym : (kCY[i] < 0 ? nd.yp : nd.y);
         const int zs = kCZ[i] > 0 ? nd.zm : (kCZ[i] < 0 ? nd.zp : nd.z);
-        int64_t off = (int64_t)i * p.stride + row_off(p, ys, zs) + nd.x;
-        if constexpr (BB_IN) {
-            if (i > 0) {
-                const uint32_t bits = kCX[i] > 0 ? bb_p : (kCX[i] < 0 ? bb_m : bb_0);
-                if ((bits >> i) & 1u)
-                    off = (int64_t)opp(i) * p.stride + row_off(p, nd.y, nd.z) +
-                          (kCX[i] > 0 ? nd.xp : (kCX[i] < 0 ? nd.xm : nd.x));
-            }
-        }
-        g[i] = ld_nt<S, T>((const S*)p.src + off);
+        g[i] = ld_nt<S, T>((const S*)p.src + (int64_t)i * p.stride + row_off(p, ys, zs) + nd.x);
     }
 #pragma unroll
     for (int i = 0; i < Q; i++)
@@ -543,9 +571,7 @@ __device__ __forceinline__ void plate_held(T* g, const Params<T>& p, const Node<
 // come from a per-node map (duct and rheometer builders). Box builders' words
 // never carry interpolated bounce-back or the rheometer's special scheme, so
 // their kernels are compiled without that code.
-// BB_DONE: the gather already delivered the bounce-back values (gather_rows
-// with BB_IN).
-template <int Q, typename T, bool MASK, bool BB_DONE = false, class PC>
+template <int Q, typename T, bool MASK, class PC>
 __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>& nd, uint32_t word, PC pc) {
     const int scheme = w_scheme(word);
     const int dir = w_dir(word);
@@ -553,15 +579,13 @@ __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>
         if (p.plates) plate_held<Q, T>(g, p, nd, pc);
     }
     if (scheme == kBounceBack) {
-        if constexpr (!BB_DONE) {
-            // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
-            if (dir >= dNW && w_geom(word)) return;
-            if (dir > dSEB) return;
-            const uint32_t m = (Q == 19) ? c_bb19.m[dir] : c_bb27.m[dir];
+        // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
+        if (dir >= dNW && w_geom(word)) return;
+        if (dir > dSEB) return;
+        const uint32_t m = (Q == 19) ? c_bb19.m[dir] : c_bb27.m[dir];
 #pragma unroll
-            for (int i = 1; i < Q; i++)
-                if (m & (1u << i)) g[i] = pc(opp(i), 0, 0, 0);
-        }
+        for (int i = 1; i < Q; i++)
+            if (m & (1u << i)) g[i] = pc(opp(i), 0, 0, 0);
     } else if (scheme == kFreeSlip) {
         // specular reflection: incoming i comes from the mirror population m
         // of the node displaced by minus the tangential part of c_i
@@ -816,6 +840,19 @@ __device__ __forceinline__ void collide_store(const Params<typename ComputeOf<S>
     }
     if (p.bgk) collide_bgk<Q, RHEO, T>(g, m, F);
     else collide<Q, RHEO, T>(g, m, F);
+    if constexpr (sizeof(S) == 2) {
+        if (p.sr) {
+            // random bits per node and step, 16 per population
+            const uint32_t base = mix32(idx ^ p.sr_seed);
+#pragma unroll
+            for (int i = 0; i < Q; i += 2) {
+                const uint32_t r = mix32(base + (uint32_t)i * 0x9e3779b9u);
+                st_nt_sr<S>(dst + (int64_t)i * p.stride + idx, g[i], r);
+                if (i + 1 < Q) st_nt_sr<S>(dst + (int64_t)(i + 1) * p.stride + idx, g[i + 1], r >> 16);
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < Q; i++) st_nt<S, T>(dst + (int64_t)i * p.stride + idx, g[i]);
 }
@@ -851,9 +888,9 @@ __device__ __forceinline__ void step_node_rows(const Params<typename ComputeOf<S
     const Node<T> nd = make_node(p, idx);
     const uint32_t word = node_word<MASK>(p, nd);
     T g[Q];
-    gather_rows<Q, S, T, !MASK>(g, p, nd, sm);
+    gather_rows<Q, S, T>(g, p, nd, sm);
     if (!in || !w_used(word)) return;
-    if (w_scheme(word) != kNull) apply_bc<Q, T, MASK, !MASK>(g, p, nd, word, GlobalPC<Q, S, T, kRepPop>{p, nd});
+    if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, GlobalPC<Q, S, T, kRepPop>{p, nd});
     collide_store<Q, S, RHEO, IBM, OUT>(p, idx, nd, g);
 }
 

@@ -12,7 +12,7 @@ namespace vlbm {
 template <typename S>
 struct Inst {
     using T = typename ComputeOf<S>::type;
-    static constexpr bool kHasMom = !std::is_same<S, _Float16>::value;
+    static constexpr bool kHasMom = sizeof(S) > 2;
     static dim3 grid_of(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 
     template <int Q, int RHEO, bool MASK, bool IBM>

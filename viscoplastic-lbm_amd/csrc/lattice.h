@@ -147,6 +147,8 @@ struct Params {
     int aux_mode;               // aux kernel: AuxMode
     int direct;                 // aux kernel: state is post-stream (no gather, no BC)
     uint32_t aux_base, aux_count;   // aux kernel: node range (count 0: every node, or every node of the extra plane)
+    int sr;                     // 16-bit storage: stochastic rounding of the stored populations
+    uint32_t sr_seed;           // its random stream of this step
     int aux_f64;                // kAuxMaterialize: doubles into dst, planes of aux_count values
     T duct_R, duct_r, w_out, u_out, w_in, u_in;
     // immersed boundary: nodes in 8^3 tiles flagged by the last IBM stage take

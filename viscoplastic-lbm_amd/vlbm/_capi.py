@@ -15,7 +15,8 @@ LIB_PATH = Path(os.environ.get("VLBM_LIB", _PKG / "lib" / "libvlbm.so"))
 VLBM_OK = 0
 ERRORS = {1: "invalid argument", 2: "device error", 3: "allocation failed", 4: "I/O error", 5: "unsupported"}
 
-FP64, FP32, FP16 = 8, 4, 2
+FP64, FP32, FP16, BF16 = 8, 4, 2, 3
+ROUND_STOCHASTIC, ROUND_NEAREST = 0, 1
 LAYOUT_AUTO, LAYOUT_POPULATIONS, LAYOUT_MOMENTS = 0, 1, 2
 NEWTONIAN, BINGHAM, POWERLAW = 0, 1, 2
 
@@ -41,6 +42,7 @@ class VlbmDesc(C.Structure):
         ("collision", C.c_int),
         ("les_const", C.c_double),
         ("layout", C.c_int),
+        ("rounding", C.c_int),
     ]
 
 

@@ -24,6 +24,7 @@ file); here they are optional extra keys with the reference defaults:
     collision: regularized                # or bgk (collisionSchemes/recursiveCollisionForceNoReg.txt)
     les_const: 0.0                        # LES_MODEL Implicit_const, Newtonian only (lbm.cu:226-242)
     layout: auto                          # state in HBM: auto | populations | moments
+    rounding: stochastic                  # 16-bit storage (half, bfloat16): stochastic | nearest
     checkpoint_save: 0                    # CHECKPOINT_SAVE interval (var.h:69-71)
     load_checkpoint: null                 # folder to resume from: LOAD_CHECKPOINT with
                                           # SIMULATION_FOLDER_LOAD_CHECKPOINT (var.h:77-84)
@@ -56,7 +57,8 @@ BUILDERS = {
     "parallelPlatesRheometer": 14,
 }
 
-PRECISIONS = {"double": _capi.FP64, "single": _capi.FP32, "half": _capi.FP16}
+PRECISIONS = {"double": _capi.FP64, "single": _capi.FP32, "half": _capi.FP16, "bfloat16": _capi.BF16}
+ROUNDINGS = {"stochastic": 0, "nearest": 1}
 MODELS = {None: _capi.NEWTONIAN, "BINGHAM": _capi.BINGHAM, "POWERLAW": _capi.POWERLAW}
 COLLISIONS = {"regularized": 0, "bgk": 1}
 LAYOUTS = {"auto": 0, "populations": 1, "moments": 2}
@@ -111,6 +113,7 @@ class SimConfig:
     collision: str = "regularized"
     les_const: float = 0.0
     layout: str = "auto"
+    rounding: str = "stochastic"
     checkpoint_save: int = 0
     load_checkpoint: Optional[str] = None
     # immersed boundary (use_ibm): vlbm_ibm_desc fields (ibmVar.h /
@@ -179,6 +182,7 @@ class SimConfig:
         d.collision = COLLISIONS[self.collision]
         d.les_const = self.les_const
         d.layout = LAYOUTS[self.layout]
+        d.rounding = ROUNDINGS[self.rounding]
         return d
 
     def replace(self, **kw) -> "SimConfig":
@@ -220,10 +224,11 @@ def _validate(c: SimConfig) -> None:
     need(c.checkpoint_save >= 0, "checkpoint_save must be >= 0")
     need(c.collision in COLLISIONS, "collision must be 'regularized' or 'bgk'")
     need(c.layout in LAYOUTS, "layout must be 'auto', 'populations' or 'moments'")
+    need(c.rounding in ROUNDINGS, "rounding must be 'stochastic' or 'nearest'")
     need(c.les_const >= 0, "les_const must be >= 0")
     need(c.les_const == 0 or c.non_newtonian_model is None, "LES applies to Newtonian fluids only")
     need(c.load_checkpoint is None or isinstance(c.load_checkpoint, str), "load_checkpoint must be a folder name")
-    need(c.precision in PRECISIONS, "precision must be 'single' or 'double' (or 'half')")
+    need(c.precision in PRECISIONS, "precision must be 'single' or 'double' (or 'half' / 'bfloat16')")
     need(c.vel_set in ("D3Q19", "D3Q27"), "vel_set must be 'D3Q19' or 'D3Q27'")
     need(c.non_newtonian_model in MODELS, "non_newtonian_model must be null, BINGHAM or POWERLAW")
     need(c.builder in BUILDERS, f"unknown builder {c.builder!r}")

@@ -1,5 +1,6 @@
 """Precision study: D3Q27 regularized power-law channel on one GPU in FP64,
-FP32 and FP16-deviation storage.
+FP32, and FP16- and bfloat16-deviation storage, each 16-bit type with
+stochastic rounding and with rounding to nearest.
 
 Case: n = 0.6, consistency K chosen so that the nominal tau is 0.7 at the
 mean shear rate (K gamma^(n-1) = (0.7 - 1/2)/3 with gamma = u_max/h, solved
@@ -84,8 +85,9 @@ def force_for_peak(h, umax, n=N_INDEX, tau=TAU_NOMINAL):
     return F, K
 
 
-def make_desc(n, precision, K, ny=None, nxz=None, F=FX):
+def make_desc(n, precision, K, ny=None, nxz=None, F=FX, rounding=_capi.ROUND_STOCHASTIC):
     d = _capi.default_desc()
+    d.rounding = rounding
     d.nx = d.ny = d.nz = n
     if ny is not None:
         d.ny = ny
@@ -112,8 +114,8 @@ def initial_fields(n, seed=4242):
             rng.uniform(-1e-4, 1e-4, shape))
 
 
-def run(n, steps, precision, K, init, log):
-    lat = Lattice(make_desc(n, precision, K))
+def run(n, steps, precision, K, init, log, rounding=_capi.ROUND_STOCHASTIC):
+    lat = Lattice(make_desc(n, precision, K, rounding=rounding))
     lat.init(*init)
     lat.step(10)
     lat.sync()
@@ -138,10 +140,10 @@ def profiles(m, n, F=FX):
     return ux, stress
 
 
-def validation(precision, steps=64000, ny=32, nxz=4, umax=0.02):
+def validation(precision, rounding=_capi.ROUND_STOCHASTIC, steps=64000, ny=32, nxz=4, umax=0.02):
     h = ny / 2.0
     F, K = force_for_peak(h, umax)
-    lat = Lattice(make_desc(0, precision, K, ny=ny, nxz=nxz, F=F))
+    lat = Lattice(make_desc(0, precision, K, ny=ny, nxz=nxz, F=F, rounding=rounding))
     lat.init()
     lat.step(steps, save_last=True)
     lat.sync()
@@ -174,9 +176,12 @@ def main():
     init = initial_fields(n, a.seed)
     results = {}
     ref = None
-    precisions = (("fp64", _capi.FP64), ("fp32", _capi.FP32), ("fp16_dev", _capi.FP16))
-    for name, prec in precisions:
-        m, mlups = run(n, a.steps, prec, K, init, log)
+    SR, RN = _capi.ROUND_STOCHASTIC, _capi.ROUND_NEAREST
+    precisions = (("fp64", _capi.FP64, RN), ("fp32", _capi.FP32, RN), ("fp16_dev_stochastic", _capi.FP16, SR),
+                  ("fp16_dev_nearest", _capi.FP16, RN), ("bf16_dev_stochastic", _capi.BF16, SR),
+                  ("bf16_dev_nearest", _capi.BF16, RN))
+    for name, prec, rnd in precisions:
+        m, mlups = run(n, a.steps, prec, K, init, log, rnd)
         entry = {"mlups": round(mlups, 1), "max_ux": float(m["ux"].max())}
         if ref is None:
             ref = m
@@ -194,7 +199,7 @@ def main():
            "lattice": [n, n, n], "steps": a.steps, "n_index": N_INDEX, "K": K,
            "tau_clamp": list(TAU_CLAMP), "F_x": FX, "results": results}
     if not a.no_validation:
-        out["converged_channel_32"] = {name: validation(prec) for name, prec in precisions}
+        out["converged_channel_32"] = {name: validation(prec, rnd) for name, prec, rnd in precisions}
         log(f"[precision_study] validation: {out['converged_channel_32']}")
     print(json.dumps(out))
 

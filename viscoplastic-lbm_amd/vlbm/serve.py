@@ -21,6 +21,7 @@ omega = omega_p).
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import ctypes as C
 import json
 import math
@@ -243,6 +244,7 @@ class SweepResult:
     latencies: List[float] = field(default_factory=list)
     errors: List[float] = field(default_factory=list)
     states: Dict[str, int] = field(default_factory=dict)
+    records: List[Dict] = field(default_factory=list)      # one per job, in completion order
 
     def summary(self) -> Dict:
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
@@ -286,6 +288,9 @@ def sweep(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every=1000
                     res.errors.append(profile_error(spec, out["ux"]))
                     res.node_updates += float(spec.nx * spec.ny * spec.nz) * inf["steps"]
                     res.states[inf["state"]] = res.states.get(inf["state"], 0) + 1
+                    res.records.append({"job": k, "state": inf["state"], "steps": inf["steps"],
+                                        "residual": inf["residual"], "latency_s": res.latencies[-1],
+                                        "profile_err": res.errors[-1], "nodes": spec.nx * spec.ny * spec.nz})
                     if keep is not None:
                         keep[k] = {"info": inf, "ux": out["ux"], "rho": out["rho"]}
                     del ids[jid]
@@ -342,15 +347,29 @@ def sequential(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every
         res.errors.append(profile_error(spec, m["ux"]))
         res.node_updates += float(spec.nx * spec.ny * spec.nz) * steps
         res.states[state] = res.states.get(state, 0) + 1
+        res.records.append({"job": k, "state": state, "steps": steps, "residual": r,
+                            "latency_s": res.latencies[-1], "profile_err": res.errors[-1],
+                            "nodes": spec.nx * spec.ny * spec.nz})
         if log:
             log(f"sequential job {k} {state} after {steps} steps")
     res.seconds = time.perf_counter() - t_start
     return res
 
 
+def stream_slice(jobs: List[JobSpec], first: int, count: int) -> List[JobSpec]:
+    """Jobs [first, first + count) of a stream, arrivals shifted so the slice
+    starts at its first job's arrival."""
+    part = jobs[first:first + count]
+    t0 = part[0].arrival if part else 0.0
+    return [dataclasses.replace(j, arrival=j.arrival - t0) for j in part]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=200)
+    ap.add_argument("--first", type=int, default=0,
+                    help="run jobs [first, first + jobs) of the seeded stream (a part of a longer sweep)")
+    ap.add_argument("--stream", type=int, default=0, help="length of the seeded stream (default: first + jobs)")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--max-steps", type=int, default=5000)
@@ -359,27 +378,40 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--tune-cache", default=None)
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--sequential-only", action="store_true")
+    ap.add_argument("--records", default=None, help="write one JSON line per job to this file")
     ap.add_argument("--resident-nodes", type=float, default=float(1 << 24),
                     help="admit while the resident lattices hold at most this many nodes (0: no limit)")
     a = ap.parse_args()
-    jobs = make_jobs(a.jobs, a.rate, a.seed)
+    jobs = stream_slice(make_jobs(max(a.stream, a.first + a.jobs), a.rate, a.seed), a.first, a.jobs)
     progress = lambda s: print(s, flush=True)   # noqa: E731
-    t = time.perf_counter()
-    res = sweep(jobs, a.resid, a.max_steps, a.check_every, realtime=True,
-                hbm_budget=int(a.budget_gb * (1 << 30)), max_jobs=256, tune_cache=a.tune_cache,
-                max_resident_nodes=int(a.resident_nodes), log=lambda s: None)
     out = {"metric": "serving sweep: jobs/s, aggregate MLUPS, p50/p99 time from arrival to result",
-           "batched": res.summary(), "batch_config": res.config,
-           "workload": {"jobs": a.jobs, "rate_per_s": a.rate, "seed": a.seed, "max_steps": a.max_steps,
-                        "resid_max": a.resid, "check_every": a.check_every,
+           "workload": {"jobs": a.jobs, "first": a.first, "rate_per_s": a.rate, "seed": a.seed,
+                        "max_steps": a.max_steps, "resid_max": a.resid, "check_every": a.check_every,
                         "max_resident_nodes": int(a.resident_nodes),
                         "lattice": "Nx U[32,256], Ny U[32,128], Nz U[16,128]; Bn U[0,5], tau U[0.55,1.5], "
                                    "Fx U[1e-6,1e-5]; D3Q19 FP32 Bingham channel"}}
-    progress(f"batched sweep done in {time.perf_counter() - t:.1f} s")
+    records = {}
+    if not a.sequential_only:
+        t = time.perf_counter()
+        res = sweep(jobs, a.resid, a.max_steps, a.check_every, realtime=True,
+                    hbm_budget=int(a.budget_gb * (1 << 30)), max_jobs=256, tune_cache=a.tune_cache,
+                    max_resident_nodes=int(a.resident_nodes), log=progress)
+        out["batched"] = res.summary()
+        out["batch_config"] = res.config
+        records["batched"] = res.records
+        progress(f"batched sweep done in {time.perf_counter() - t:.1f} s")
     if not a.no_sequential:
-        seq = sequential(jobs, a.resid, a.max_steps, a.check_every)
+        seq = sequential(jobs, a.resid, a.max_steps, a.check_every, log=progress)
         out["sequential"] = seq.summary()
-        out["speedup_jobs_per_s"] = round(out["batched"]["jobs_per_s"] / out["sequential"]["jobs_per_s"], 2)
+        records["sequential"] = seq.records
+        if "batched" in out:
+            out["speedup_jobs_per_s"] = round(out["batched"]["jobs_per_s"] / out["sequential"]["jobs_per_s"], 2)
+    if a.records:
+        with open(a.records, "w") as f:
+            for mode, recs in records.items():
+                for r in recs:
+                    f.write(json.dumps(dict(r, mode=mode, job=r["job"] + a.first)) + "\n")
     print(json.dumps(out))
 
 
