@@ -258,7 +258,7 @@ class SweepResult:
 
 
 def sweep(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every=1000, realtime=True,
-          keep=None, log=None, **batch_kw) -> SweepResult:
+          keep=None, log=None, sink=None, **batch_kw) -> SweepResult:
     """Continuous batching of a job stream: arrivals are submitted when their
     time comes (wall clock from the start) while earlier jobs run."""
     b = Batch(check_every=check_every, **batch_kw)
@@ -290,7 +290,10 @@ def sweep(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every=1000
                     res.states[inf["state"]] = res.states.get(inf["state"], 0) + 1
                     res.records.append({"job": k, "state": inf["state"], "steps": inf["steps"],
                                         "residual": inf["residual"], "latency_s": res.latencies[-1],
-                                        "profile_err": res.errors[-1], "nodes": spec.nx * spec.ny * spec.nz})
+                                        "profile_err": res.errors[-1], "nodes": spec.nx * spec.ny * spec.nz,
+                                        "t_done_s": t_done})
+                    if sink:
+                        sink(res.records[-1])
                     if keep is not None:
                         keep[k] = {"info": inf, "ux": out["ux"], "rho": out["rho"]}
                     del ids[jid]
@@ -306,7 +309,7 @@ def sweep(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every=1000
 
 
 def sequential(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every=1000, log=None,
-               realtime=True) -> SweepResult:
+               realtime=True, sink=None) -> SweepResult:
     """The same jobs one at a time through the plain single-lattice path,
     with the same stop test and check cadence (first step direct, then
     checks after 1 + k * check_every steps), in arrival order: a job starts
@@ -349,7 +352,9 @@ def sequential(jobs: List[JobSpec], resid_max=1e-6, max_steps=50000, check_every
         res.states[state] = res.states.get(state, 0) + 1
         res.records.append({"job": k, "state": state, "steps": steps, "residual": r,
                             "latency_s": res.latencies[-1], "profile_err": res.errors[-1],
-                            "nodes": spec.nx * spec.ny * spec.nz})
+                            "nodes": spec.nx * spec.ny * spec.nz, "t_done_s": t_done})
+        if sink:
+            sink(res.records[-1])
         if log:
             log(f"sequential job {k} {state} after {steps} steps")
     res.seconds = time.perf_counter() - t_start
@@ -391,27 +396,32 @@ def main():
                         "max_resident_nodes": int(a.resident_nodes),
                         "lattice": "Nx U[32,256], Ny U[32,128], Nz U[16,128]; Bn U[0,5], tau U[0.55,1.5], "
                                    "Fx U[1e-6,1e-5]; D3Q19 FP32 Bingham channel"}}
-    records = {}
+    rec_file = open(a.records, "w") if a.records else None
+
+    def sink_for(mode):
+        if rec_file is None:
+            return None
+
+        def sink(r):   # one line per finished job, written as it finishes
+            rec_file.write(json.dumps(dict(r, mode=mode, job=r["job"] + a.first)) + "\n")
+            rec_file.flush()
+        return sink
+
     if not a.sequential_only:
         t = time.perf_counter()
         res = sweep(jobs, a.resid, a.max_steps, a.check_every, realtime=True,
                     hbm_budget=int(a.budget_gb * (1 << 30)), max_jobs=256, tune_cache=a.tune_cache,
-                    max_resident_nodes=int(a.resident_nodes), log=progress)
+                    max_resident_nodes=int(a.resident_nodes), log=progress, sink=sink_for("batched"))
         out["batched"] = res.summary()
         out["batch_config"] = res.config
-        records["batched"] = res.records
         progress(f"batched sweep done in {time.perf_counter() - t:.1f} s")
     if not a.no_sequential:
-        seq = sequential(jobs, a.resid, a.max_steps, a.check_every, log=progress)
+        seq = sequential(jobs, a.resid, a.max_steps, a.check_every, log=progress, sink=sink_for("sequential"))
         out["sequential"] = seq.summary()
-        records["sequential"] = seq.records
         if "batched" in out:
             out["speedup_jobs_per_s"] = round(out["batched"]["jobs_per_s"] / out["sequential"]["jobs_per_s"], 2)
-    if a.records:
-        with open(a.records, "w") as f:
-            for mode, recs in records.items():
-                for r in recs:
-                    f.write(json.dumps(dict(r, mode=mode, job=r["job"] + a.first)) + "\n")
+    if rec_file:
+        rec_file.close()
     print(json.dumps(out))
 
 
