@@ -60,24 +60,42 @@ def initial_fields(seed=12345):
     return rho, ux, uy, uz
 
 
-def cpu_baseline(init, target_s=15.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(init, target_s=15.0, fp64_s=8.0):
     """The oracle (OpenMP restatement of the reference per-node path, FP32
     storage as the reference's SINGLE_PRECISION build) on the host cores,
-    on a bounded number of steps of the same 256^3 workload."""
+    on a bounded number of steps of the same 256^3 workload; beside it a
+    shorter sample of the FP64 (DOUBLE_PRECISION) build."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
-    cfg = workload_config("single")
-    o = po.Oracle(po.desc_from(cfg.to_desc()), 4)
-    o.init(*init)
     threads = po.max_threads()
-    t1 = o.time_steps(1, threads)           # first touch + one step
-    n = max(2, min(200, int(target_s / max(t1, 1e-3))))
-    t = o.time_steps(n, threads)
-    o.close()
-    mlups = N_SIDE ** 3 * n / t / 1e6
+    out = {}
+    for prec, budget in ((4, target_s), (8, fp64_s)):
+        cfg = workload_config("single" if prec == 4 else "double")
+        o = po.Oracle(po.desc_from(cfg.to_desc()), prec)
+        o.init(*init)
+        t1 = o.time_steps(1, threads)           # first touch + one step
+        n = max(2, min(200, int(budget / max(t1, 1e-3))))
+        t = o.time_steps(n, threads)
+        o.close()
+        out[prec] = (N_SIDE ** 3 * n / t / 1e6, n, t)
+    mlups, n, t = out[4]
+    m64, n64, t64 = out[8]
     return {"value": round(mlups, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{n} steps of the 256^3 D3Q19 FP32 Bingham workload in {t:.1f} s "
-                      f"(oracle OpenMP loop, collapse(2) over z,y)"}
+                      f"(oracle OpenMP loop, collapse(2) over z,y)",
+            "fp64": {"value": round(m64, 2), "unit": "MLUPS",
+                     "sample": f"{n64} steps of the same workload with FP64 storage in {t64:.1f} s"}}
 
 
 SWEEP_RATE, SWEEP_SEED, SWEEP_RESID, SWEEP_MAX_STEPS, SWEEP_CHECK = 20.0, 7, 1e-6, 50000, 1000

@@ -106,6 +106,9 @@ typedef struct vlbm_desc {
     double les_const;        /* LES_MODEL's Implicit_const (lbm.cu:226-242); 0 = no LES, Newtonian only */
     int layout;              /* VLBM_LAYOUT_* (not in the reference: its state is always populations) */
     int rounding;            /* VLBM_ROUND_*: rounding of 16-bit storage (not in the reference) */
+    int les_visc_turb;       /* LES_EXPORT_VISC_TURBULENT (LES/lesVar.h:16): keep the eddy viscosity
+                                tau_t/3 of every node (lbm.cu:229-239), read with vlbm_visc_turb and
+                                saved as ID_visc_turb<step>.bin; needs les_const > 0 */
 } vlbm_desc;
 
 typedef struct vlbm_ctx* vlbm_h;
@@ -183,6 +186,10 @@ int vlbm_compute_macroscopics(vlbm_h h);
 /* Copies the saved macroscopics to host doubles, [NZ][NY][NX]; any pointer
  * may be NULL. omega is the apparent omega (non-Newtonian only). */
 int vlbm_macroscopics(vlbm_h h, double* rho, double* ux, double* uy, double* uz, double* omega);
+
+/* LES eddy viscosity tau_t/3 of the last step, NX*NY*NZ doubles (macr.visc_turb,
+ * written every step when desc.les_visc_turb is set; 0 before the first). */
+int vlbm_visc_turb(vlbm_h h, double* out);
 
 /* Residual and mean density of the saved macroscopics with FP64
  * accumulation: residual = sum|u - u_old| / sum|u| (1 when the denominator is

@@ -105,6 +105,7 @@ struct or_desc {
     double outer_rotation, outer_velocity, inner_rotation, inner_velocity;
     int collision;         // 0 regularized (lbm.cu:263-461), 1 BGK (recursiveCollisionForceNoReg.txt)
     double les_const;      // LES_MODEL Implicit_const (lbm.cu:226-242); 0 = no LES
+    int les_visc_turb;     // LES_EXPORT_VISC_TURBULENT (lesVar.h:16): keep tau_t/3 per node
 };
 
 } // extern "C"
@@ -401,6 +402,7 @@ struct Lattice {
     std::vector<uint32_t> map;
     std::vector<size_t> bc_list;
     std::vector<R> rho, ux, uy, uz, omega;
+    std::vector<R> visc_turb;   // LES_EXPORT_VISC_TURBULENT: macr.visc_turb
     std::vector<R> force_x, force_y, force_z;   // per-node force (IBM builds only)
     R w[27];
 
@@ -416,6 +418,7 @@ struct Lattice {
         popAux.assign(nplane * q, R(0));
         rho.assign(nnodes, R(0)); ux.assign(nnodes, R(0)); uy.assign(nnodes, R(0));
         uz.assign(nnodes, R(0)); omega.assign(nnodes, R(0));
+        visc_turb.assign(nnodes, R(0));   // INIT_VISC_TURB (lbmInitialization.cu:114-115) is 0
         map.resize(nnodes);
         for (int z = 0; z < nz; z++)
             for (int y = 0; y < ny; y++)
@@ -579,6 +582,7 @@ struct Lattice {
                 // LES_MODEL (lbm.cu:226-242): implicit Smagorinsky eddy relaxation
                 const R TAU = (R)d.tau;
                 const R tau_t = 0.5 * std::sqrt(TAU * TAU + (R)d.les_const * stressMag) - 0.5 * TAU;
+                m_out[5] = tau_t / 3.0;   // visc_turb_var (lbm.cu:229)
                 omegaVar = 1.0 / (TAU + tau_t);
             }
         }
@@ -1105,7 +1109,7 @@ struct Lattice {
         #pragma omp parallel for collapse(2) schedule(static) num_threads(nthreads)
         for (int z = 0; z < nz; z++)
             for (int y = 0; y < ny; y++) {
-                R f[27], mo[5];
+                R f[27], mo[6];
                 const int yp1 = (y + 1) % ny, ym1 = (ny + y - 1) % ny;
                 const int zp1 = (z + 1) % (nz + 1), zm1 = ((nz + 1) + z - 1) % (nz + 1);
                 for (int x = 0; x < nx; x++) {
@@ -1125,6 +1129,8 @@ struct Lattice {
                         rho[k] = mo[0]; ux[k] = mo[1]; uy[k] = mo[2]; uz[k] = mo[3];
                         if (d.rheo == 1) omega[k] = mo[4];
                     }
+                    // written every step, as the kernel does (lbm.cu:237-239)
+                    if (d.les_const != 0 && d.les_visc_turb) visc_turb[k] = mo[5];
                     if (w_save_post_col(m))
                         for (int i = 0; i < q; i++) pop[ip(x, y, z, i)] = f[i];
                     const int xp1 = (x + 1) % nx, xm1 = (nx + x - 1) % nx;
@@ -1810,6 +1816,10 @@ void or_get_macr(void* h, double* rho, double* ux, double* uy, double* uz, doubl
         if (omega) omega[k] = L->omega[k];
     });
 }
+// macr.visc_turb (LES_EXPORT_VISC_TURBULENT), [NZ][NY][NX].
+void or_get_visc_turb(void* h, double* out) {
+    OR_DISPATCH(h, for (size_t k = 0; k < L->nnodes; k++) out[k] = L->visc_turb[k]);
+}
 void or_set_omega(void* h, const double* omega) {
     OR_DISPATCH(h, for (size_t k = 0; k < L->nnodes; k++) L->omega[k] = omega[k]);
 }
@@ -1819,7 +1829,7 @@ void or_collide_node(void* h, const double* fin, double fx, double fy, double fz
                      double omega_old, double* fout, double* m) {
     OR_DISPATCH(h, {
         using R = typename std::remove_reference<decltype(L->pop[0])>::type;
-        R f[27], mo[5]; R om = (R)omega_old;
+        R f[27], mo[6]; R om = (R)omega_old;
         for (int i = 0; i < L->q; i++) f[i] = (R)fin[i];
         L->collide(f, (R)fx, (R)fy, (R)fz, &om, mo);
         for (int i = 0; i < L->q; i++) fout[i] = f[i];

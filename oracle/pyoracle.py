@@ -35,7 +35,7 @@ class OrDesc(C.Structure):
         ("duct_outer_radius", C.c_double), ("duct_inner_radius", C.c_double),
         ("outer_rotation", C.c_double), ("outer_velocity", C.c_double),
         ("inner_rotation", C.c_double), ("inner_velocity", C.c_double),
-        ("collision", C.c_int), ("les_const", C.c_double),
+        ("collision", C.c_int), ("les_const", C.c_double), ("les_visc_turb", C.c_int),
     ]
 
 
@@ -93,6 +93,7 @@ def lib():
         L.or_max_threads.restype = C.c_int
         L.or_get_macr.argtypes = [C.c_void_p, dp, dp, dp, dp, dp]
         L.or_set_omega.argtypes = [C.c_void_p, dp]
+        L.or_get_visc_turb.argtypes = [C.c_void_p, dp]
         L.or_collide_node.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
         L.or_apply_bc_raw.argtypes = [C.c_void_p, dp, dp]
         L.or_set_map.argtypes = [C.c_void_p, P(C.c_uint32)]
@@ -198,6 +199,11 @@ class Oracle:
             aux = np.ascontiguousarray(pop_aux, dtype=np.float64)
             assert aux.shape == pop.shape
             lib().or_set_pops2(self.h, _p(pop), _p(aux))
+
+    def visc_turb(self) -> np.ndarray:
+        out = np.zeros(self.shape)
+        lib().or_get_visc_turb(self.h, _p(out))
+        return out
 
     def set_omega(self, omega):
         lib().or_set_omega(self.h, _p(np.ascontiguousarray(omega, dtype=np.float64)))

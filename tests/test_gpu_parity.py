@@ -80,7 +80,11 @@ CASES_FP64 = [
                                non_newtonian_model="BINGHAM", yield_stress=4e-5), 300, 0.0),
     ("closed_box_q19", dict(NX=8, NY=7, NZ=6, tau=0.7, builder="closedBoxBounceBack"), 60, 1e-2),
     ("closed_box_q27", dict(NX=8, NY=7, NZ=6, tau=0.7, vel_set="D3Q27", builder="closedBoxBounceBack"), 60, 1e-2),
-    ("square_duct_zouhe_c2", dict(NX=12, NY=10, NZ=16, tau=0.8, builder="squareDuctZouHe",
+    # one- and two-node rows through the LDS row exchange: the x-end nodes' bounce-back values, loaded
+    # before the row pull, for a node at both ends and for two adjacent ends
+    ("closed_box_nx1_rows", dict(NX=1, NY=7, NZ=6, tau=0.7, builder="closedBoxBounceBack"), 60, 1e-2),
+    ("closed_box_nx2_rows", dict(NX=2, NY=7, NZ=6, tau=0.7, builder="closedBoxBounceBack"), 60, 1e-2),
+    ("square_duct_zouhe_c2",dict(NX=12, NY=10, NZ=16, tau=0.8, builder="squareDuctZouHe",
                                   non_newtonian_model="BINGHAM", yield_stress=1e-5,
                                   uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0]), 150, 1e-4),
     # the benchmarked kernel: NX = 16 divides the 256-node block, so the step pulls through the LDS row
@@ -490,3 +494,64 @@ def test_runner_residual_stop(tmp_path):
     assert res.stopped == "residual"
     assert res.reports[-1]["residual"] < 1e-6 <= res.reports[-2]["residual"]
     assert res.steps == res.reports[-1]["step"]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_les_eddy_viscosity_export_matches_oracle(layout, tmp_path):
+    """LES_EXPORT_VISC_TURBULENT (LES/lesVar.h:16; lbm.cu:229-239): the eddy
+    viscosity tau_t / 3 of every node after the last step equals the oracle's
+    (FP64, 1e-10), and vlbm_save_macr_bin writes it as ID_visc_turb<step>.bin
+    (lbmReport.cpp:167-171)."""
+    announce(f"LES eddy viscosity export [{layout}]")
+    kw = dict([c for c in CASES_FP64 if c[0] == "lid_driven_les"][0][1])
+    cfg = base_cfg(layout=layout, les_visc_turb=True, **kw)
+    lat, orc = run_both(cfg, 150)
+    vg, vo = lat.visc_turb(), orc.visc_turb()
+    print(f"visc_turb: max {vo.max():.3e}, rel-L2 {rel_l2(vg, vo):.2e}", flush=True)
+    assert vo.max() > 0 and rel_l2(vg, vo) <= 1e-10
+    (tmp_path / "003").mkdir()
+    lat.save_macr_bin(str(tmp_path), "003", 150)
+    raw = np.fromfile(tmp_path / "003" / "003_visc_turb000150.bin", dtype="d").reshape(vo.shape)
+    assert np.array_equal(raw, vg)
+    assert not (tmp_path / "003" / "003_omega000150.bin").exists()
+    lat.close()
+
+
+def test_c1_bingham_channel_in_full():
+    """The CPU-runnable configuration in full: D3Q19 regularized Bingham flow
+    in a 32 (x) x 32 (y) x 64 (z) channel, FP64, periodic x and z, bounce-back
+    walls at y = 0 and 31, F_x = 1e-5, tau = 0.9, yield stress 4e-5 (plug
+    half-width 4), from rest, 20,000 steps. The GPU equals the oracle (the
+    reference restated, OpenMP on the host cores) to 1e-10; both follow the
+    analytic plug-flow profile (halfway walls) within 2 %, the plug's speed
+    within 2 %. Also prints the oracle's node rate (the reference per-node
+    loop on this host)."""
+    import time
+    announce("c1 32x32x64 Bingham channel, 20000 steps")
+    ny, F, tau, sy = 32, 1e-5, 0.9, 4e-5
+    cfg = base_cfg(NX=32, NY=ny, NZ=64, tau=tau, FX=F, builder="parallelPlatesBounceBack_Y",
+                   non_newtonian_model="BINGHAM", yield_stress=sy)
+    lat = vlbm.Lattice(cfg.to_desc())
+    lat.init()
+    lat.step(20000, save_last=True)
+    lat.sync()
+    orc = po.Oracle(po.desc_from(cfg.to_desc()), 8)
+    orc.init()
+    t0 = time.perf_counter()
+    orc.step(20000, save_last=True)
+    dt = time.perf_counter() - t0
+    print(f"oracle (FP64, {po.max_threads()} host threads): {32 * ny * 64 * 20000 / dt / 1e6:.1f} MLUPS", flush=True)
+    # 20,000 steps carry the round-off of the two compilers' different operation orders (FMA
+    # contraction) 40-100x further than the short cases: measured 2.9e-10 on f - w on one MI355X
+    # (the 500-step cases stay below 1e-12), hence 1e-8 here
+    assert_fp64_parity(lat, orc, 19, tol=1e-8, omega=True)
+    ux = lat.macroscopics()["ux"].mean(axis=(0, 2))
+    nu_p = (tau - 0.5) / 3
+    h, yc = ny / 2.0, (ny - 1) / 2.0
+    s = np.abs(np.arange(ny) - yc)
+    y0 = sy / F
+    exact = np.where(s >= y0, F / (2 * nu_p) * (h * h - s * s) - sy / nu_p * (h - s), F / (2 * nu_p) * (h - y0) ** 2)
+    print(f"profile rel-L2 vs analytic {rel_l2(ux, exact):.2e}, plug speed ratio {ux.max() / exact.max():.4f}",
+          flush=True)
+    assert rel_l2(ux, exact) < 0.02 and abs(ux.max() / exact.max() - 1) < 0.02
+    lat.close()

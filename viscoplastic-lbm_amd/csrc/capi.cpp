@@ -222,6 +222,8 @@ int validate(const vlbm_desc* d) {
     if (d->les_const > 0 && d->rheology != VLBM_NEWTONIAN)
         return fail(VLBM_ERR_UNSUPPORTED, "LES applies to Newtonian fluids only");
     if (d->les_const > 0 && !(d->tau > 0.5)) return fail(VLBM_ERR_INVALID, "LES needs tau > 0.5");
+    if (d->les_visc_turb && !(d->les_const > 0))
+        return fail(VLBM_ERR_INVALID, "the eddy viscosity export needs LES (les_const > 0)");
     if (d->layout < VLBM_LAYOUT_AUTO || d->layout > VLBM_LAYOUT_MOMENTS) return fail(VLBM_ERR_INVALID, "unknown layout");
     if (d->layout == VLBM_LAYOUT_MOMENTS && (d->collision != VLBM_COLLISION_REGULARIZED || d->precision == VLBM_FP16 ||
                                              d->precision == VLBM_BF16))
@@ -375,6 +377,14 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
         if ((rc = dalloc(h, &h->uold[k], h->n * h->tbytes))) { release(h); return rc; }
     if (d->rheology == VLBM_POWERLAW)
         if ((rc = dalloc(h, &h->omega_state, h->n * h->tbytes))) { release(h); return rc; }
+    if (d->les_visc_turb) {
+        // INIT_VISC_TURB = 0 (LES/lesVar.h:21, lbmInitialization.cu:114-115)
+        if ((rc = dalloc(h, &h->visc_turb, h->n * h->tbytes))) { release(h); return rc; }
+        if (hipMemsetAsync(h->visc_turb, 0, h->n * h->tbytes, h->stream) != hipSuccess) {
+            release(h);
+            return fail(VLBM_ERR_DEVICE, "hipMemset");
+        }
+    }
     if (h->mask_geom) {
         if ((rc = dalloc(h, (void**)&h->d_mask, h->n * 4))) { release(h); return rc; }
         e = hipMemcpy(h->d_mask, h->words.data(), h->n * 4, hipMemcpyHostToDevice);
@@ -809,6 +819,12 @@ int vlbm_macroscopics(vlbm_h h, double* rho, double* ux, double* uy, double* uz,
     return VLBM_OK;
 }
 
+int vlbm_visc_turb(vlbm_h h, double* out) {
+    if (!h || !out) return fail(VLBM_ERR_INVALID, "handle or output NULL");
+    if (!h->visc_turb) return fail(VLBM_ERR_INVALID, "the handle keeps no eddy viscosity (desc.les_visc_turb)");
+    return fetch_field(h, h->visc_turb, out);
+}
+
 int vlbm_reduce(vlbm_h h, double* residual, double* mean_rho) {
     if (!h) return fail(VLBM_ERR_INVALID, "handle is NULL");
     if (launch_reduce(h->tbytes, h->macr[0], h->macr[1], h->macr[2], h->macr[3], h->uold[0], h->uold[1],
@@ -837,11 +853,14 @@ static std::string var_filename(const char* path, const char* id, const char* va
 
 int vlbm_save_macr_bin(vlbm_h h, const char* path, const char* id, unsigned step) {
     if (!h || !path || !id) return fail(VLBM_ERR_INVALID, "NULL argument");
-    const char* names[5] = {"rho", "ux", "uy", "uz", "omega"};
-    const int nfields = h->d.rheology == VLBM_NEWTONIAN ? 4 : 5;
+    // saveAllMacrBin (lbmReport.cpp:133-172): rho, u, omega (non-Newtonian),
+    // visc_turb (LES_EXPORT_VISC_TURBULENT)
+    const char* names[6] = {"rho", "ux", "uy", "uz", "omega", "visc_turb"};
+    const void* fields[6] = {h->macr[0], h->macr[1], h->macr[2], h->macr[3], h->macr[4], h->visc_turb};
     std::vector<double> buf(h->n);
-    for (int k = 0; k < nfields; k++) {
-        int rc = fetch_field(h, h->macr[k], buf.data());
+    for (int k = 0; k < 6; k++) {
+        if (!fields[k] || (k == 4 && h->d.rheology == VLBM_NEWTONIAN)) continue;
+        int rc = fetch_field(h, fields[k], buf.data());
         if (rc) return rc;
         const std::string fn = var_filename(path, id, names[k], step);
         FILE* f = std::fopen(fn.c_str(), "wb");

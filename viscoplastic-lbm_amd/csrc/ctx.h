@@ -59,6 +59,7 @@ struct vlbm_ctx {
     uint32_t region_word[27];
     bool mask_geom = false;
     void* omega_state = nullptr;
+    void* visc_turb = nullptr;    // LES eddy viscosity per node (desc.les_visc_turb)
     void* macr[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // rho, ux, uy, uz, omega
     void* uold[3] = {nullptr, nullptr, nullptr};
     double* partials = nullptr;
@@ -125,6 +126,7 @@ inline void fill_params(const vlbm_ctx* h, Params<T>& p) {
     p.first_bc = (h->lbm_steps == 1 && !h->resumed) ? 1 : 0;
     p.mask = h->d_mask;
     p.omega_state = (T*)h->omega_state;
+    p.visc_turb = (T*)h->visc_turb;
     p.m_rho = (T*)h->macr[0]; p.m_ux = (T*)h->macr[1]; p.m_uy = (T*)h->macr[2];
     p.m_uz = (T*)h->macr[3]; p.m_omega = (T*)h->macr[4];
     p.tile_flag = h->tile_flag;
@@ -172,6 +174,7 @@ inline void release(vlbm_ctx* h) {
     if (!h) return;
     ibm_release(h);
     if (h->pops) (void)hipFree(h->pops);
+    if (h->visc_turb) (void)hipFree(h->visc_turb);
     if (h->scratch_pops) (void)hipFree(h->scratch_pops);
     for (void* p : h->lat) if (p) (void)hipFree(p);
     for (void* p : h->macr) if (p) (void)hipFree(p);

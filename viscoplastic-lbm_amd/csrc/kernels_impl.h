@@ -443,6 +443,13 @@ struct GlobalPC {
     }
 };
 
+// The D3Q19 population pairs (c_x = +1, c_x = -1): slot of population i
+// among them (-1 for c_x = 0).
+__host__ __device__ constexpr int xpair_slot(int i) {
+    return (i == 1 || i == 2) ? 0 : (i == 7 || i == 8) ? 1 : (i == 9 || i == 10) ? 2
+         : (i == 13 || i == 14) ? 3 : (i == 15 || i == 16) ? 4 : -1;
+}
+
 // Gather (pull streaming) of the post-stream deviations: g_i(x) = g*_i(x - c_i).
 template <int Q, typename T, class PC>
 __device__ __forceinline__ void gather(T* g, PC pc) {
@@ -631,6 +638,7 @@ __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>
 template <int Q, int RHEO, typename T>
 struct Moments {
     T dr, rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, omega;
+    T tau_t;     // LES eddy relaxation time (0 without LES)
 };
 
 // Body force acting on a node: the constant force, or the per-node field of
@@ -681,6 +689,7 @@ __device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<
     m.pxz = sxz - rho * ux * uz;
     m.pyz = syz - rho * uy * uz;
     m.dr = dr; m.rho = rho; m.ux = ux; m.uy = uy; m.uz = uz;
+    m.tau_t = 0;
     T om = p.omega;
     if (RHEO != 0 || p.les_c != (T)0) {
         // |Pi^neq + sym(u F)| (src/CUDA/lbm.cu:192-208)
@@ -693,6 +702,7 @@ __device__ __forceinline__ Moments<Q, RHEO, T> moments(const T* g, const Params<
             // implicit Smagorinsky closure (LES_MODEL, src/CUDA/lbm.cu:226-242):
             // tau_t = (sqrt(tau^2 + C |S|) - tau) / 2, omega = 1 / (tau + tau_t)
             const T tau_t = (T)0.5 * sqrt(p.tau * p.tau + p.les_c * smag) - (T)0.5 * p.tau;
+            m.tau_t = tau_t;
             om = (T)1 / (p.tau + tau_t);
         } else if (RHEO == 1) {
             // Bingham: omega = omega_p max(0, 1 - tau_y/|S|) (src/CUDA/NNF/nnf.h:83);
@@ -826,6 +836,8 @@ __device__ __forceinline__ void collide_store(const Params<typename ComputeOf<S>
     const T om_old = (RHEO == 2) ? p.omega_state[idx] : (T)0;
     const Moments<Q, RHEO, T> m = moments<Q, RHEO, T>(g, p, om_old, F);
     if (RHEO == 2) p.omega_state[idx] = m.omega;
+    // LES_EXPORT_VISC_TURBULENT: every step, as the reference (lbm.cu:237-239)
+    if (RHEO == 0 && p.visc_turb) p.visc_turb[idx] = m.tau_t / (T)3;
     if (p.save) {
         p.m_rho[idx] = m.rho; p.m_ux[idx] = m.ux; p.m_uy[idx] = m.uy; p.m_uz[idx] = m.uz;
         if (RHEO != 0) p.m_omega[idx] = m.omega;
@@ -888,9 +900,42 @@ __device__ __forceinline__ void step_node_rows(const Params<typename ComputeOf<S
     const Node<T> nd = make_node(p, idx);
     const uint32_t word = node_word<MASK>(p, nd);
     T g[Q];
-    gather_rows<Q, S, T>(g, p, nd, sm);
-    if (!in || !w_used(word)) return;
-    if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, GlobalPC<Q, S, T, kRepPop>{p, nd});
+    if constexpr (!MASK && Q == 19) {
+        // the x-end nodes (x = 0, NX-1) load, before the row pull, the own
+        // post-collision populations a bounce-back on the W (c_x = -1 ones)
+        // or E face (c_x = +1 ones) takes: in the row pull they would be a
+        // second round of loads after the barrier, for 2 nodes of every row,
+        // which the whole block waits for. With NX = 1 the node is both ends:
+        // it prefetches the W face's values and loads the E face's in the BC.
+        const GlobalPC<Q, S, T, kRepPop> pc{p, nd};
+        const bool xw = nd.x == 0, xe = nd.x == p.nx - 1 && !xw;
+        T v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0;
+        if (xw) { v0 = pc(2, 0, 0, 0); v1 = pc(8, 0, 0, 0); v2 = pc(10, 0, 0, 0); v3 = pc(14, 0, 0, 0); v4 = pc(16, 0, 0, 0); }
+        if (xe) { v0 = pc(1, 0, 0, 0); v1 = pc(7, 0, 0, 0); v2 = pc(9, 0, 0, 0); v3 = pc(13, 0, 0, 0); v4 = pc(15, 0, 0, 0); }
+        gather_rows<Q, S, T>(g, p, nd, sm);
+        if (!in || !w_used(word)) return;
+        const int scheme = w_scheme(word), dir = w_dir(word);
+        if (scheme == kBounceBack) {
+            // convex edges/corners carry no bounce-back (boundaryConditionsHandler.cu:158-256)
+            if (!(dir >= dNW && w_geom(word)) && dir <= dSEB) {
+                const uint32_t m = c_bb19.m[dir];
+#pragma unroll
+                for (int i = 1; i < Q; i++) {
+                    if (!(m & (1u << i))) continue;
+                    const int j = opp(i), slot = xpair_slot(j);
+                    const bool pre = slot >= 0 && (kCX[j] < 0 ? xw : xe);
+                    const T vs = slot == 0 ? v0 : slot == 1 ? v1 : slot == 2 ? v2 : slot == 3 ? v3 : v4;
+                    g[i] = pre ? vs : pc(j, 0, 0, 0);
+                }
+            }
+        } else if (scheme != kNull) {
+            apply_bc<Q, T, MASK>(g, p, nd, word, pc);
+        }
+    } else {
+        gather_rows<Q, S, T>(g, p, nd, sm);
+        if (!in || !w_used(word)) return;
+        if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, GlobalPC<Q, S, T, kRepPop>{p, nd});
+    }
     collide_store<Q, S, RHEO, IBM, OUT>(p, idx, nd, g);
 }
 

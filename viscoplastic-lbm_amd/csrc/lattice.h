@@ -144,6 +144,7 @@ struct Params {
     int zouhe_textbook;
     int bgk;                 // BGK collision instead of the regularized one
     T tau, les_c;            // LES: tau_t = (sqrt(tau^2 + les_c |S|) - tau) / 2 when les_c != 0
+    T* visc_turb;            // LES: tau_t / 3 of every node, every step (null: not kept)
     int aux_mode;               // aux kernel: AuxMode
     int direct;                 // aux kernel: state is post-stream (no gather, no BC)
     uint32_t aux_base, aux_count;   // aux kernel: node range (count 0: every node, or every node of the extra plane)

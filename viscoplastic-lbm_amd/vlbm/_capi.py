@@ -43,6 +43,7 @@ class VlbmDesc(C.Structure):
         ("les_const", C.c_double),
         ("layout", C.c_int),
         ("rounding", C.c_int),
+        ("les_visc_turb", C.c_int),
     ]
 
 
@@ -95,6 +96,7 @@ def _load() -> C.CDLL:
         "vlbm_get_pops": (C.c_int, [C.c_void_p, dp]),
         "vlbm_get_pops_noalloc": (C.c_int, [C.c_void_p, dp]),
         "vlbm_set_map": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+        "vlbm_visc_turb": (C.c_int, [C.c_void_p, dp]),
         "vlbm_set_omega": (C.c_int, [C.c_void_p, dp]),
         "vlbm_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "vlbm_compute_macroscopics": (C.c_int, [C.c_void_p]),

@@ -23,6 +23,7 @@ file); here they are optional extra keys with the reference defaults:
     zouhe_textbook: false                 # textbook Zou-He instead of the reference lines
     collision: regularized                # or bgk (collisionSchemes/recursiveCollisionForceNoReg.txt)
     les_const: 0.0                        # LES_MODEL Implicit_const, Newtonian only (lbm.cu:226-242)
+    les_visc_turb: false                  # LES_EXPORT_VISC_TURBULENT: keep and save tau_t/3 per node
     layout: auto                          # state in HBM: auto | populations | moments
     rounding: stochastic                  # 16-bit storage (half, bfloat16): stochastic | nearest
     checkpoint_save: 0                    # CHECKPOINT_SAVE interval (var.h:69-71)
@@ -112,6 +113,7 @@ class SimConfig:
     zouhe_textbook: bool = False
     collision: str = "regularized"
     les_const: float = 0.0
+    les_visc_turb: bool = False
     layout: str = "auto"
     rounding: str = "stochastic"
     checkpoint_save: int = 0
@@ -181,6 +183,7 @@ class SimConfig:
         d.zouhe_textbook = 1 if self.zouhe_textbook else 0
         d.collision = COLLISIONS[self.collision]
         d.les_const = self.les_const
+        d.les_visc_turb = 1 if self.les_visc_turb else 0
         d.layout = LAYOUTS[self.layout]
         d.rounding = ROUNDINGS[self.rounding]
         return d
@@ -219,7 +222,7 @@ def _validate(c: SimConfig) -> None:
                 pass
         need(isinstance(v, (int, float)) and not isinstance(v, bool), f"{name} must be a number")
         setattr(c, name, float(v))
-    for name in ("use_ibm", "data_stop", "data_save", "pop_save", "zouhe_textbook"):
+    for name in ("use_ibm", "data_stop", "data_save", "pop_save", "zouhe_textbook", "les_visc_turb"):
         need(isinstance(getattr(c, name), bool), f"{name} must be a boolean")
     need(c.checkpoint_save >= 0, "checkpoint_save must be >= 0")
     need(c.collision in COLLISIONS, "collision must be 'regularized' or 'bgk'")
@@ -227,6 +230,7 @@ def _validate(c: SimConfig) -> None:
     need(c.rounding in ROUNDINGS, "rounding must be 'stochastic' or 'nearest'")
     need(c.les_const >= 0, "les_const must be >= 0")
     need(c.les_const == 0 or c.non_newtonian_model is None, "LES applies to Newtonian fluids only")
+    need(not c.les_visc_turb or c.les_const > 0, "les_visc_turb needs LES (les_const > 0)")
     need(c.load_checkpoint is None or isinstance(c.load_checkpoint, str), "load_checkpoint must be a folder name")
     need(c.precision in PRECISIONS, "precision must be 'single' or 'double' (or 'half' / 'bfloat16')")
     need(c.vel_set in ("D3Q19", "D3Q27"), "vel_set must be 'D3Q19' or 'D3Q27'")

@@ -97,6 +97,12 @@ class Lattice:
         check(lib.vlbm_get_pops(self._h, _ptr(out)))
         return out
 
+    def visc_turb(self) -> np.ndarray:
+        """LES eddy viscosity tau_t / 3 of the last step ([NZ][NY][NX])."""
+        out = np.zeros(self.shape, dtype=np.float64)
+        check(lib.vlbm_visc_turb(self._h, _ptr(out)))
+        return out
+
     def set_map(self, words: np.ndarray) -> None:
         """Replaces the node map with words written outside the library
         ([NZ][NY][NX] uint32 in the layout of build_map)."""
