@@ -678,10 +678,21 @@ __device__ __forceinline__ RowWeights row_weights(int bx, int by, int bz, double
 // sums, wave64 tree), added to the sphere by k_ibm_add_chunks.
 constexpr int kRowLanes = 16;
 
+// Workgroups go to the 8 XCDs of the device in turn (blockIdx mod 8), each
+// XCD with its own L2. Block b takes chunk (b mod 8) ceil(n / 8) + b / 8, so
+// consecutive chunks (one sphere's) run on one XCD and share its L2 copy of
+// the sphere's stage records. The grid is 8 ceil(n / 8) blocks.
+__device__ __forceinline__ int xcd_chunk(int n_chunks) {
+    const int per = (n_chunks + 7) / 8;
+    return (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+}
+inline uint32_t xcd_chunk_grid(int n_chunks) { return 8u * (uint32_t)((n_chunks + 7) / 8); }
+
 template <typename T>
 __global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFluid<T> fl) {
     __shared__ double red[6][kInterpBlock / 64];
-    const int chunk = blockIdx.x;
+    const int chunk = xcd_chunk(P.n_chunks);
+    if (chunk >= P.n_chunks) return;
     const IbmSphere& c = P.sph[P.chunk_sphere[chunk]];
     const int first = P.chunk_first[chunk];
     const int r = threadIdx.x & (kRowLanes - 1);

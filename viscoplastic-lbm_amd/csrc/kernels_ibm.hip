@@ -51,7 +51,7 @@ static int stage_part(const IbmParams& P, const IbmFluid<T>& fl, int part, hipSt
         break;
     case kIbmIteration:
         if (nn) {
-            hipLaunchKernelGGL((k_ibm_interp<T>), dim3((uint32_t)P.n_chunks), dim3(kInterpBlock), 0, st, P, fl);
+            hipLaunchKernelGGL((k_ibm_interp<T>), dim3(xcd_chunk_grid(P.n_chunks)), dim3(kInterpBlock), 0, st, P, fl);
             hipLaunchKernelGGL(k_ibm_add_chunks, blocks_for(ns), blk, 0, st, P);
         }
         if (ns) hipLaunchKernelGGL(k_ibm_velocity, blocks_for(ns), blk, 0, st, P);
