@@ -17,7 +17,7 @@ constexpr int kMaxColl = 13;        // history slots: 7 walls + 6 partners (MAX_
 constexpr int kFirstPartnerSlot = 7;
 constexpr int kMaxContacts = 16;    // sphere-sphere contacts recorded per sphere and step
 constexpr int kIbmBlock = 256;
-constexpr int kMaxSortKeys = 4096;  // spheres handled by the one-block cell sort
+constexpr int kMaxSortKeys = 4096;  // spheres handled by the one-block cell sort (keys held in LDS)
 
 // One sphere. The first 48 doubles are the C-ABI particle record
 // (vlbm_ibm_get_particles) in this order.
@@ -71,6 +71,7 @@ struct IbmParams {
     uint32_t* tile_count;
     int cells[3];
     double cell_w[3];
+    uint64_t* cell_keys;                   // [n_spheres] (cell << 32 | sphere), ascending
     IbmContact* contacts;                  // [n_spheres][kMaxContacts]
     int* n_contacts;
     int* error;

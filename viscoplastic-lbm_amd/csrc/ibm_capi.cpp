@@ -383,6 +383,7 @@ int vlbm_ibm_enable(vlbm_h h, const vlbm_ibm_desc* d) {
     A((void**)&P.tile_flag, (size_t)P.n_tiles * 4);
     A((void**)&P.tile_list, (size_t)P.n_tiles * 4);
     A((void**)&P.tile_count, 4);
+    A((void**)&P.cell_keys, ms * sizeof(uint64_t));
     A((void**)&P.contacts, ms * kMaxContacts * sizeof(IbmContact));
     A((void**)&P.n_contacts, ms * 4);
     const size_t mc = mn / kChunkNodes + ms + 1;

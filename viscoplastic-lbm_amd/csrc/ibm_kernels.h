@@ -6,10 +6,10 @@
 //             a tile list (first touch wins an atomic flag)
 //   macr      rho, u of the post-step populations with the constant force on
 //             the listed tiles only; force field reset to the constant there
-//   contacts  one workgroup: uniform cell list (bitonic sort of (cell, sphere)
+//   contacts  uniform cell list (one-workgroup bitonic sort of (cell, sphere)
 //             in LDS), Hertz-Mindlin sphere-sphere contacts by the lower-index
-//             sphere, force / torque sums per sphere in a fixed order, walls,
-//             velocity update
+//             sphere and force / torque sums per sphere in a fixed order, one
+//             wave per sphere; walls; velocity update
 //   iteration (x max_iterations) one workgroup per sphere: interpolation and
 //             force of each node, FP64 sums of the node forces / moments
 //             (fixed-order tree); velocity update; spreading of the node
@@ -178,8 +178,8 @@ __global__ __launch_bounds__(1024) void k_ibm_list_tiles(IbmParams P) {
 // The distinct cells whose spheres can touch a sphere of the given cell, as
 // one index range per axis: axis a visits lo[a] + k (mod the cell count when
 // periodic) for k < cnt[a]. Kept as ranges, not a list: an indexed list of
-// 27 cells would live in private (scratch) memory, whose every access waits
-// on a memory round trip in the one-workgroup contact stage.
+// 27 cells would live in private (scratch) memory, each lane of a contact
+// wave computes its one cell from the ranges.
 struct CellRange {
     int lo[3], cnt[3];
 };
@@ -404,36 +404,6 @@ __device__ void wall_contact(const IbmParams& P, IbmSphere& c, const double wall
     M[2] += r_i * (n[0] * ft[1] - n[1] * ft[0]);
 }
 
-// sphere-sphere contacts of sphere i with higher-index spheres of the
-// neighbouring cells, in ascending partner order (the reference's pair order
-// for that sphere's history)
-__device__ void sphere_contacts(const IbmParams& P, const uint64_t* keys, int i) {
-    IbmSphere& ci = P.sph[i];
-    int cand[kMaxContacts];
-    int nc = 0;
-    const CellRange nb = neighbour_range(P, ci.cell);
-    for (int c = 0; c < nb.cnt[0] * nb.cnt[1] * nb.cnt[2]; c++) {
-        const uint64_t cell = range_cell(P, nb, c % nb.cnt[0], (c / nb.cnt[0]) % nb.cnt[1], c / (nb.cnt[0] * nb.cnt[1]));
-        for (int k = lower_bound_key(keys, P.n_spheres, cell << 32); k < P.n_spheres && (keys[k] >> 32) == cell; k++) {
-            const int j = (int)(keys[k] & 0xffffffffu);
-            if (j <= i) continue;
-            const IbmSphere& cj = P.sph[j];
-            if (!ci.movable && !cj.movable) continue;
-            if (!(sphere_gap(P, ci, cj) < 0)) continue;
-            if (nc == kMaxContacts) { atomicOr(P.error, (int)kIbmErrContacts); continue; }
-            int at = nc++;
-            while (at > 0 && cand[at - 1] > j) { cand[at] = cand[at - 1]; at--; }
-            cand[at] = j;
-        }
-    }
-    int rec = 0;
-    for (int k = 0; k < nc; k++) {
-        IbmContact ct;
-        if (pair_contact(P, ci, P.sph[cand[k]], cand[k], P.step, ct)) P.contacts[i * kMaxContacts + rec++] = ct;
-    }
-    P.n_contacts[i] = rec;
-}
-
 // checkCollisionWallsSphere (ibmCollision.cu:188-251): walls at 0 and N-1
 __device__ void wall_contacts(const IbmParams& P, IbmSphere& c, double f[3], double M[3]) {
     for (int a = 0; a < 3; a++) {
@@ -448,34 +418,155 @@ __device__ void wall_contacts(const IbmParams& P, IbmSphere& c, double f[3], dou
     }
 }
 
-// force / torque of the contacts in the reference's accumulation order for
-// sphere p: as the higher index (ascending partner), as the lower index
-// (ascending partner), then the walls
-__device__ void sphere_contact_sum(const IbmParams& P, const uint64_t* keys, int p) {
-    IbmSphere& cp = P.sph[p];
-    double f[3] = {cp.f[0], cp.f[1], cp.f[2]};
-    double M[3] = {cp.M[0], cp.M[1], cp.M[2]};
-    int from_i[kMaxContacts], from_k[kMaxContacts];
-    int nr = 0;
-    const CellRange nb = neighbour_range(P, cp.cell);
-    for (int c = 0; c < nb.cnt[0] * nb.cnt[1] * nb.cnt[2]; c++) {
-        const uint64_t cell = range_cell(P, nb, c % nb.cnt[0], (c / nb.cnt[0]) % nb.cnt[1], c / (nb.cnt[0] * nb.cnt[1]));
-        for (int k = lower_bound_key(keys, P.n_spheres, cell << 32); k < P.n_spheres && (keys[k] >> 32) == cell; k++) {
-            const int i = (int)(keys[k] & 0xffffffffu);
-            if (i >= p) continue;
-            const int nci = P.n_contacts[i];
-            for (int r = 0; r < nci; r++) {
-                if (P.contacts[i * kMaxContacts + r].j != p) continue;
-                if (nr == kMaxContacts) { atomicOr(P.error, (int)kIbmErrPartners); continue; }
-                int at = nr++;
-                while (at > 0 && from_i[at - 1] > i) { from_i[at] = from_i[at - 1]; from_k[at] = from_k[at - 1]; at--; }
-                from_i[at] = i;
-                from_k[at] = r;
+// The contact kernels run one wave per sphere. Each lane of the wave takes
+// one cell of the sphere's neighbour range (at most 27) and walks that cell's
+// run of the sorted keys; what the lanes find is handed to lane 0 by
+// shuffles, in rounds the whole wave takes together, and lane 0 keeps it in a
+// list sorted by sphere index in LDS. Lane 0 then does the order-dependent
+// work (history updates, sums) exactly as one thread per sphere would.
+constexpr int kContactWaves = 4;                  // waves (spheres) per workgroup
+constexpr int kContactBlock = 64 * kContactWaves;
+
+// The sorted keys into LDS, by every thread of the workgroup.
+__device__ __forceinline__ void load_cell_keys(const IbmParams& P, uint64_t* sk) {
+    for (int k = threadIdx.x; k < P.n_spheres; k += kContactBlock) sk[k] = P.cell_keys[k];
+    __syncthreads();
+}
+
+// This lane's run of keys [k, kend): cell `lane` of the range around `cell`.
+__device__ __forceinline__ void lane_cell_run(const IbmParams& P, const uint64_t* sk, int cell, int lane, int& k,
+                                              int& kend) {
+    const CellRange nb = neighbour_range(P, cell);
+    k = kend = 0;
+    if (lane < nb.cnt[0] * nb.cnt[1] * nb.cnt[2]) {
+        const uint64_t c = range_cell(P, nb, lane % nb.cnt[0], (lane / nb.cnt[0]) % nb.cnt[1],
+                                      lane / (nb.cnt[0] * nb.cnt[1]));
+        k = lower_bound_key(sk, P.n_spheres, c << 32);
+        kend = lower_bound_key(sk, P.n_spheres, (c + 1) << 32);
+    }
+}
+
+// Inserts v (and w beside it, when lw is given) into the list of n values
+// kept ascending, at most kMaxContacts long: when full, the largest value is
+// dropped and `over` set.
+__device__ __forceinline__ void insert_sorted(int* lv, int* lw, int& n, int v, int w, bool& over) {
+    if (n == kMaxContacts) {
+        over = true;
+        if (v > lv[n - 1]) return;
+        n--;
+    }
+    int at = n++;
+    while (at > 0 && lv[at - 1] > v) {
+        lv[at] = lv[at - 1];
+        if (lw) lw[at] = lw[at - 1];
+        at--;
+    }
+    lv[at] = v;
+    if (lw) lw[at] = w;
+}
+
+// Cell sort: one workgroup, bitonic sort of the (cell, sphere) keys in LDS,
+// written to cell_keys.
+__global__ __launch_bounds__(1024) void k_ibm_cell_sort(IbmParams P) {
+    __shared__ uint64_t key[kMaxSortKeys];
+    const int n = P.n_spheres;
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = threadIdx.x; i < m; i += 1024)
+        key[i] = i < n ? (((uint64_t)(uint32_t)P.sph[i].cell << 32) | (uint32_t)i) : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += 1024) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & k) == 0;
+                    const uint64_t a = key[i], b = key[ixj];
+                    if ((a > b) == up) { key[i] = b; key[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < n; i += 1024) P.cell_keys[i] = key[i];
+}
+
+// Sphere-sphere contacts of sphere i with the higher-index spheres of the
+// neighbouring cells, in ascending partner order (the reference's pair order
+// for that sphere's history); one wave per sphere.
+__global__ __launch_bounds__(kContactBlock) void k_ibm_pairs(IbmParams P) {
+    __shared__ uint64_t sk[kMaxSortKeys];
+    __shared__ int cand[kContactWaves][kMaxContacts];
+    load_cell_keys(P, sk);
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int i = (int)blockIdx.x * kContactWaves + wave;
+    if (i >= P.n_spheres) return;
+    IbmSphere& ci = P.sph[i];
+    int k, kend;
+    lane_cell_run(P, sk, ci.cell, lane, k, kend);
+    int nc = 0;
+    bool over = false;
+    while (__ballot(k < kend)) {
+        int j = -1;
+        if (k < kend) {
+            const int jj = (int)(sk[k++] & 0xffffffffu);
+            if (jj > i) {
+                const IbmSphere& cj = P.sph[jj];
+                if ((ci.movable || cj.movable) && sphere_gap(P, ci, cj) < 0) j = jj;
             }
         }
+        for (uint64_t m = __ballot(j >= 0); m; m &= m - 1) {
+            const int jb = __shfl(j, (int)__builtin_ctzll(m), 64);
+            if (lane == 0) insert_sorted(cand[wave], nullptr, nc, jb, 0, over);
+        }
     }
+    if (lane != 0) return;
+    if (over) atomicOr(P.error, (int)kIbmErrContacts);
+    int rec = 0;
+    for (int c = 0; c < nc; c++) {
+        const int j = cand[wave][c];
+        IbmContact ct;
+        if (pair_contact(P, ci, P.sph[j], j, P.step, ct)) P.contacts[i * kMaxContacts + rec++] = ct;
+    }
+    P.n_contacts[i] = rec;
+}
+
+// Force / torque of the contacts in the reference's accumulation order for
+// sphere p: as the higher index (ascending partner), as the lower index
+// (ascending partner), then the walls; one wave per sphere.
+__global__ __launch_bounds__(kContactBlock) void k_ibm_contact_sums(IbmParams P) {
+    __shared__ uint64_t sk[kMaxSortKeys];
+    __shared__ int from_i[kContactWaves][kMaxContacts], from_k[kContactWaves][kMaxContacts];
+    load_cell_keys(P, sk);
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int p = (int)blockIdx.x * kContactWaves + wave;
+    if (p >= P.n_spheres) return;
+    IbmSphere& cp = P.sph[p];
+    int k, kend;
+    lane_cell_run(P, sk, cp.cell, lane, k, kend);
+    int nr = 0;
+    bool over = false;
+    while (__ballot(k < kend)) {
+        int fi = -1, fk = -1;
+        if (k < kend) {
+            const int i = (int)(sk[k++] & 0xffffffffu);
+            if (i < p) {
+                const int nci = P.n_contacts[i];
+                for (int r = 0; r < nci; r++)
+                    if (P.contacts[i * kMaxContacts + r].j == p) { fi = i; fk = r; break; }
+            }
+        }
+        for (uint64_t m = __ballot(fi >= 0); m; m &= m - 1) {
+            const int b = (int)__builtin_ctzll(m);
+            const int ib = __shfl(fi, b, 64), kb = __shfl(fk, b, 64);
+            if (lane == 0) insert_sorted(from_i[wave], from_k[wave], nr, ib, kb, over);
+        }
+    }
+    if (lane != 0) return;
+    if (over) atomicOr(P.error, (int)kIbmErrPartners);
+    double f[3] = {cp.f[0], cp.f[1], cp.f[2]};
+    double M[3] = {cp.M[0], cp.M[1], cp.M[2]};
     for (int r = 0; r < nr; r++) {
-        const IbmContact& ct = P.contacts[from_i[r] * kMaxContacts + from_k[r]];
+        const IbmContact& ct = P.contacts[from_i[wave][r] * kMaxContacts + from_k[wave][r]];
         for (int a = 0; a < 3; a++) { f[a] += ct.fd[a]; M[a] += ct.mj[a]; }
     }
     const int own = P.n_contacts[p];
@@ -545,37 +636,6 @@ __device__ void sphere_velocity(const IbmParams& P, int s) {
 __global__ __launch_bounds__(kIbmBlock) void k_ibm_velocity(IbmParams P) {
     const int s = blockIdx.x * kIbmBlock + threadIdx.x;
     if (s < P.n_spheres) sphere_velocity(P, s);
-}
-
-// Contact part of the stage in one workgroup: (cell, sphere) keys sorted in
-// LDS (bitonic), contacts by the lower-index sphere, sums per sphere, velocity.
-__global__ __launch_bounds__(1024) void k_ibm_contact_stage(IbmParams P) {
-    __shared__ uint64_t key[kMaxSortKeys];
-    const int n = P.n_spheres;
-    int m = 1;
-    while (m < n) m <<= 1;
-    for (int i = threadIdx.x; i < m; i += 1024)
-        key[i] = i < n ? (((uint64_t)(uint32_t)P.sph[i].cell << 32) | (uint32_t)i) : ~0ull;
-    __syncthreads();
-    for (int k = 2; k <= m; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < m; i += 1024) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const bool up = (i & k) == 0;
-                    const uint64_t a = key[i], b = key[ixj];
-                    if ((a > b) == up) { key[i] = b; key[ixj] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    for (int i = threadIdx.x; i < n; i += 1024) sphere_contacts(P, key, i);
-    __threadfence_block();
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 1024) sphere_contact_sum(P, key, i);
-    __threadfence_block();
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += 1024) sphere_velocity(P, i);
 }
 
 // ---------------------------------------------------------------------------

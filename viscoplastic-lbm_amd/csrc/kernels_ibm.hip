@@ -47,7 +47,13 @@ static int stage_part(const IbmParams& P, const IbmFluid<T>& fl, int part, hipSt
         hipLaunchKernelGGL(k_ibm_list_tiles, dim3((P.n_tiles + 1023) / 1024), dim3(1024), 0, st, P);
         break;
     case kIbmContacts:
-        if (ns) hipLaunchKernelGGL(k_ibm_contact_stage, dim3(1), dim3(1024), 0, st, P);
+        if (ns) {
+            const dim3 waves((ns + kContactWaves - 1) / kContactWaves);
+            hipLaunchKernelGGL(k_ibm_cell_sort, dim3(1), dim3(1024), 0, st, P);
+            hipLaunchKernelGGL(k_ibm_pairs, waves, dim3(kContactBlock), 0, st, P);
+            hipLaunchKernelGGL(k_ibm_contact_sums, waves, dim3(kContactBlock), 0, st, P);
+            hipLaunchKernelGGL(k_ibm_velocity, blocks_for(ns), blk, 0, st, P);
+        }
         break;
     case kIbmIteration:
         if (nn) {
