@@ -181,6 +181,40 @@ def test_ibm_fp32_fluid_fp64_particles():
     compare_fluid(lat, o, 1e-4)
 
 
+def test_ibm_fp32_plain_flush_equals_atomic_flush():
+    """The spreading flush writes box nodes that no other sphere's box covers
+    with plain stores, the others with atomics. With boxes that do not overlap,
+    every node has one contributor, so the plain run equals, bit for bit, a
+    run whose stray node sends every box to the atomic flush; both follow the
+    FP64 oracle within the FP32 bars."""
+    print("case: fp32 three isolated spheres, plain vs atomic flush, 10 steps", flush=True)
+    out = []
+    for stray in (False, True):
+        lat, o = make_pair(64, 32, 32, "parallelPlatesBounceBack_Y", dict(gy=-2e-5), precision="single", FX=1e-5)
+        for k, x in enumerate((12.0, 32.0, 52.0)):
+            c = [x, 15.0 + 0.3 * k, 16.0 + 0.2 * k]
+            xyz, S = po.sphere_polar(8.0, c)
+            if stray and k == 1:
+                xyz = np.vstack([xyz, [c[0] + 6.0, c[1], c[2]]])
+                S = np.append(S, 0.0)
+            add_both(lat, o, c, 4.0, vel=(0.001 * (k - 1), 0, 0), w=(0, 0, 0.002 * k), xyz=xyz, S=S)
+        lat.step(10, save_last=True)
+        o.ibm_run(10, 0)
+        lat.sync()
+        m = lat.macroscopics()
+        out.append((m, lat.ibm_force(), lat.ibm_particles()))
+        g, c = lat.ibm_particles(), o.ibm_particles()
+        assert rel(g[:, F["pos"]], c[:, F["pos"]]) <= 1e-6
+        assert rel(g[:, F["vel"]], c[:, F["vel"]]) <= 1e-3
+        compare_fluid(lat, o, 1e-4)
+        lat.close()
+    (m0, f0, p0), (m1, f1, p1) = out
+    for k in ("rho", "ux", "uy", "uz"):
+        assert np.array_equal(m0[k], m1[k]), k
+    assert np.array_equal(f0, f1)
+    assert np.array_equal(p0[:, F["pos"]], p1[:, F["pos"]]) and np.array_equal(p0[:, F["vel"]], p1[:, F["vel"]])
+
+
 def test_ibm_timing_split_and_status():
     from vlbm import _capi
     lat, o = make_pair(32, 32, 32, "closedBoxBounceBack",

@@ -291,9 +291,7 @@ int ibm_read(vlbm_ctx* h, const std::string& pre, int tb, int border) {
                 for (int x = 0; x < h->d.nx; x++, k++) {
                     const double fv = h->tbytes == 8 ? f[k] : (double)(float)f[k];
                     if (fv != f0t) {
-                        const uint32_t t = (uint32_t)(x >> kTileShift) +
-                                           (uint32_t)P.tiles[0] * ((uint32_t)(y >> kTileShift) +
-                                                                   (uint32_t)P.tiles[1] * (uint32_t)(z >> kTileShift));
+                        const uint32_t t = tile_index(x, y, z, P.tiles[0], P.tiles[1]);
                         flags[t] = 1;
                     }
                 }

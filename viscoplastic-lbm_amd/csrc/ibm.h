@@ -17,6 +17,7 @@ constexpr int kMaxColl = 13;        // history slots: 7 walls + 6 partners (MAX_
 constexpr int kFirstPartnerSlot = 7;
 constexpr int kMaxContacts = 16;    // sphere-sphere contacts recorded per sphere and step
 constexpr int kIbmBlock = 256;
+constexpr int kBoxNb = 8;           // overlapping spreading boxes listed per sphere
 constexpr int kMaxSortKeys = 4096;  // spheres handled by the one-block cell sort (keys held in LDS)
 
 // One sphere. The first 48 doubles are the C-ABI particle record
@@ -76,6 +77,10 @@ struct IbmParams {
     int* n_contacts;
     int* error;
     int box_edge;                          // LDS spreading box edge (0: global atomics)
+    int box_plain;                         // box nodes no other box covers may take plain stores
+    double box_reach;                      // ceil(r_max) + 1/2: node offsets within it stay in the box
+    int* box_nb;                           // [n_spheres][kBoxNb + 1]: count (-1: too many), overlapping boxes
+    int* box_stray;                        // 1: some node lies beyond box_reach of its centre (this stage)
     // interpolation work: chunks of at most kChunkNodes nodes of one sphere;
     // the chunks of sphere s are [chunk0[s], chunk0[s + 1])
     int n_chunks;

@@ -41,6 +41,16 @@ void update_cells(const vlbm_ctx* h, IbmState& s) {
             bytes + 4096 <= prop.sharedMemPerBlock)
             s.P.box_edge = edge;
     }
+    // plain stores into box nodes need every lattice node to appear once in a
+    // box: no periodic axis shorter than the box
+    s.P.box_plain = 0;
+    s.P.box_reach = std::ceil(s.r_max) + 0.5;
+    if (s.P.box_edge > 0) {
+        const int n3[3] = {h->d.nx, h->d.ny, h->d.nz};
+        s.P.box_plain = 1;
+        for (int a = 0; a < 3; a++)
+            if (s.P.periodic[a] && n3[a] < s.P.box_edge) s.P.box_plain = 0;
+    }
     const int n[3] = {h->d.nx, h->d.ny, h->d.nz};
     const double cs = 2.0 * s.r_max * 1.0001;
     for (int a = 0; a < 3; a++) {
@@ -357,7 +367,7 @@ int vlbm_ibm_enable(vlbm_h h, const vlbm_ibm_desc* d) {
     P.nx = h->d.nx; P.ny = h->d.ny; P.nz = h->d.nz;
     P.periodic[0] = d->periodic_x != 0; P.periodic[1] = d->periodic_y != 0; P.periodic[2] = d->periodic_z != 0;
     const int n3[3] = {P.nx, P.ny, P.nz};
-    for (int a = 0; a < 3; a++) P.tiles[a] = (n3[a] + kTileEdge - 1) / kTileEdge;
+    for (int a = 0; a < 3; a++) P.tiles[a] = tiles_along(n3[a], a);
     P.n_tiles = (uint32_t)P.tiles[0] * (uint32_t)P.tiles[1] * (uint32_t)P.tiles[2];
     P.thickness = d->thickness; P.alpha = d->alpha; P.fluid_density = d->fluid_density;
     P.g[0] = d->gx; P.g[1] = d->gy; P.g[2] = d->gz;
@@ -393,6 +403,8 @@ int vlbm_ibm_enable(vlbm_h h, const vlbm_ibm_desc* d) {
     A((void**)&P.chunk0, (ms + 1) * 4);
     A((void**)&P.chunk_sums, mc * 6 * sizeof(double));
     A((void**)&P.error, 4);
+    A((void**)&P.box_nb, ms * (kBoxNb + 1) * 4);
+    A((void**)&P.box_stray, 4);
     for (int a = 0; a < 3; a++) A(&s.force[a], h->n * h->tbytes);
     A(&s.stage, h->n * 4 * h->tbytes);
     if (rc) { ibm_release(h); return rc; }
@@ -514,8 +526,7 @@ int vlbm_ibm_get_force(vlbm_h h, double* f) {
         for (int z = 0; z < h->d.nz; z++)
             for (int y = 0; y < h->d.ny; y++)
                 for (int x = 0; x < h->d.nx; x++, k++) {
-                    const uint32_t t = (uint32_t)(x >> kTileShift) +
-                                       (uint32_t)s.P.tiles[0] * ((uint32_t)(y >> kTileShift) + (uint32_t)s.P.tiles[1] * (uint32_t)(z >> kTileShift));
+                    const uint32_t t = tile_index(x, y, z, s.P.tiles[0], s.P.tiles[1]);
                     if (!flags[t]) f[a * h->n + k] = f0[a];
                 }
     }

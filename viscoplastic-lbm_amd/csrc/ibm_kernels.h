@@ -2,7 +2,7 @@
 //
 // One stage per time step, after the LBM step (src/CUDA/main.cu:369-389):
 //   begin     old values, cell index of each sphere, node forces reset; the
-//             4^3 tiles touched by the interpolation stencils are appended to
+//             16x2x2 tiles touched by the interpolation stencils are appended to
 //             a tile list (first touch wins an atomic flag)
 //   macr      rho, u of the post-step populations with the constant force on
 //             the listed tiles only; force field reset to the constant there
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_begin(IbmParams P) {
     const uint32_t gid = blockIdx.x * kIbmBlock + threadIdx.x;
     const uint32_t stride = gridDim.x * kIbmBlock;
     for (uint32_t t = gid; t < P.n_tiles; t += stride) P.tile_flag[t] = 0;
-    if (gid == 0) *P.tile_count = 0;
+    if (gid == 0) { *P.tile_count = 0; *P.box_stray = 0; }
     if (gid >= (uint32_t)P.n_spheres) return;
     IbmSphere& c = P.sph[gid];
     int cc[3];
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_mark(IbmParams P) {
     for (int a = 0; a < 3; a++) {
         nt[a] = 0;
         for (int j = s.mn[a]; j <= s.mx[a]; j++) {
-            const int t = stencil_coord(P, s, a, j) >> kTileShift;
+            const int t = stencil_coord(P, s, a, j) >> tile_shift(a);
             bool seen = false;
             for (int k = 0; k < nt[a]; k++) seen |= (tl[a][k] == t);
             if (!seen) tl[a][nt[a]++] = t;
@@ -858,6 +858,69 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_add_chunks(IbmParams P) {
     for (int a = 0; a < 3; a++) { c.f[a] += t[a]; c.M[a] += t[3 + a]; }
 }
 
+// Origin of a sphere's spreading box on axis a (the box is box_edge nodes
+// wide, centred on the sphere).
+__device__ __forceinline__ int box_origin(const IbmParams& P, const IbmSphere& c, int a) {
+    return (int)floor(c.pos[a]) - (P.box_edge / 2 - 3) - 2;
+}
+
+// Whether lattice coordinate q (unwrapped, axis a) lies in the box of origin o.
+__device__ __forceinline__ bool in_box_axis(const IbmParams& P, int a, int q, int o) {
+    int d = q - o;
+    if (P.periodic[a]) d = wrap_axis(d, axis_n(P, a));
+    return d >= 0 && d < P.box_edge;
+}
+
+// Spreading plan of the stage (one wave per sphere; positions are fixed until
+// the stage moves the spheres): the spheres whose boxes overlap this one's
+// (at most kBoxNb listed, else the count is -1), and box_stray set when a
+// node lies farther than box_reach from its centre on some axis. Nodes within
+// box_reach put every stencil point in their own sphere's box, so with no
+// stray node a box node outside every other box receives values from its own
+// sphere's workgroup only.
+__global__ __launch_bounds__(kContactBlock) void k_ibm_box_plan(IbmParams P) {
+    const int lane = (int)threadIdx.x & 63;
+    const int s = (int)blockIdx.x * kContactWaves + ((int)threadIdx.x >> 6);
+    if (s >= P.n_spheres) return;
+    const IbmSphere& c = P.sph[s];
+    bool out = false;
+    for (int k = lane; k < c.n_nodes; k += 64) {
+        const int i = c.first_node + k;
+        const double p[3] = {P.px[i], P.py[i], P.pz[i]};
+        for (int a = 0; a < 3; a++) {
+            double d = p[a] - c.pos[a];
+            if (P.periodic[a]) {
+                const double n = (double)axis_n(P, a);
+                if (d > n / 2) d -= n;
+                if (d < -n / 2) d += n;
+            }
+            if (!(fabs(d) <= P.box_reach)) out = true;
+        }
+    }
+    if (__ballot(out) && lane == 0) atomicOr(P.box_stray, 1);
+    const int o[3] = {box_origin(P, c, 0), box_origin(P, c, 1), box_origin(P, c, 2)};
+    int* list = P.box_nb + s * (kBoxNb + 1);
+    int cnt = 0;
+    for (int j0 = 0; j0 < P.n_spheres; j0 += 64) {
+        const int j = j0 + lane;
+        bool ov = false;
+        if (j < P.n_spheres && j != s) {
+            const IbmSphere& cj = P.sph[j];
+            ov = true;
+            for (int a = 0; a < 3; a++) {
+                const int oj = box_origin(P, cj, a);
+                // two boxes of one width intersect when either origin lies in the other box
+                if (!in_box_axis(P, a, oj, o[a]) && !in_box_axis(P, a, o[a], oj)) ov = false;
+            }
+        }
+        const uint64_t m = __ballot(ov);
+        const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
+        if (ov && at < kBoxNb) list[1 + at] = j;
+        cnt += __popcll(m);
+    }
+    if (lane == 0) list[0] = cnt > kBoxNb ? -1 : cnt;
+}
+
 // Adds the spread force v to node kk of density rho: to the force field and,
 // as v / 2 rho, to the node's stage velocity (velAux of spreadForce,
 // ibm.cu:489-509).
@@ -876,6 +939,13 @@ __device__ __forceinline__ void add_spread(const IbmFluid<T>& fl, size_t kk, T r
 template <typename T>
 __device__ __forceinline__ void add_spread(const IbmFluid<T>& fl, size_t kk, T vx, T vy, T vz) {
     add_spread<T>(fl, kk, fl.stage[kk].rho, vx, vy, vz);
+}
+
+// u + v h with the product rounded before the sum, as an atomic add of the
+// rounded product gives it (no fused multiply-add)
+__device__ __forceinline__ float add_rounded_product(float u, float v, float h) {
+#pragma clang fp contract(off)
+    return u + v * h;
 }
 
 // (2a) spreading into the force field with global atomics, one wave per node
@@ -918,7 +988,14 @@ __global__ __launch_bounds__(kSphereBlock) void k_ibm_spread_box(IbmParams P, Ib
     __shared__ double red[3][kSphereBlock / 64];
     __shared__ double scale[3];
     __shared__ int finite[3];
+    __shared__ int nb_o[kBoxNb][3];                // origins of the overlapping boxes
     const IbmSphere& c = P.sph[blockIdx.x];
+    const int* plan = P.box_nb + blockIdx.x * (kBoxNb + 1);
+    const int n_nb = P.box_plain && *P.box_stray == 0 ? plan[0] : -1;   // -1: atomics everywhere
+    if ((int)threadIdx.x < n_nb) {
+        const IbmSphere& cj = P.sph[plan[1 + threadIdx.x]];
+        for (int a = 0; a < 3; a++) nb_o[threadIdx.x][a] = box_origin(P, cj, a);
+    }
     const int edge = P.box_edge;
     const int e3 = edge * edge * edge;
     for (int e = threadIdx.x; e < 3 * e3; e += kSphereBlock) box[e] = 0;
@@ -995,33 +1072,58 @@ __global__ __launch_bounds__(kSphereBlock) void k_ibm_spread_box(IbmParams P, Ib
         }
     }
     __syncthreads();
-    // flush in rounds of kFlushBatch box nodes per thread: the densities of a
-    // round are loaded before its atomics are issued, so each load waits for
-    // the previous round's atomics only (memory operations retire in order)
+    // flush in rounds of kFlushBatch box nodes per thread. A node that no
+    // other sphere's box covers (and, with use_box, no other workgroup
+    // writes: see k_ibm_box_plan) takes plain loads and stores, the others
+    // global atomics; both add the same float values, so a node's result is
+    // the same either way. The loads of a round are issued before its
+    // atomics and stores, so each waits for the previous round's only
+    // (memory operations retire in order).
     constexpr int kFlushBatch = 8;
+    const bool plain_ok = use_box && n_nb >= 0;
     for (int e0 = 0; e0 < e3; e0 += kFlushBatch * kSphereBlock) {
         size_t kk[kFlushBatch];
-        float rho[kFlushBatch];
-        bool on[kFlushBatch];
+        StageRec<float> rec[kFlushBatch];
+        float f[kFlushBatch][3];
+        bool on[kFlushBatch], plain[kFlushBatch];
 #pragma unroll
         for (int j = 0; j < kFlushBatch; j++) {
             const int e = e0 + j * kSphereBlock + (int)threadIdx.x;
             on[j] = e < e3 && (box[e] != 0 || box[e3 + e] != 0 || box[2 * e3 + e] != 0);
             int q[3] = {o[0] + e % edge, o[1] + (e / edge) % edge, o[2] + e / (edge * edge)};
+            plain[j] = plain_ok;
+            for (int k = 0; k < kBoxNb; k++)
+                if (k < n_nb && in_box_axis(P, 0, q[0], nb_o[k][0]) && in_box_axis(P, 1, q[1], nb_o[k][1]) &&
+                    in_box_axis(P, 2, q[2], nb_o[k][2]))
+                    plain[j] = false;
             for (int a = 0; a < 3; a++) {
                 const int n = axis_n(P, a);
                 if (P.periodic[a]) q[a] = wrap_axis(q[a], n);
                 else if (q[a] < 0 || q[a] >= n) on[j] = false;
             }
             kk[j] = on[j] ? ((size_t)q[2] * P.ny + q[1]) * P.nx + q[0] : 0;
-            rho[j] = on[j] ? fl.stage[kk[j]].rho : 1.f;
+            rec[j] = on[j] ? fl.stage[kk[j]] : StageRec<float>{1.f, 0.f, 0.f, 0.f};
+            if (on[j] && plain[j]) {
+                f[j][0] = fl.fx[kk[j]]; f[j][1] = fl.fy[kk[j]]; f[j][2] = fl.fz[kk[j]];
+            }
         }
 #pragma unroll
         for (int j = 0; j < kFlushBatch; j++) {
             const int e = e0 + j * kSphereBlock + (int)threadIdx.x;
-            if (on[j])
-                add_spread<float>(fl, kk[j], rho[j], (float)((double)box[e] / sx), (float)((double)box[e3 + e] / sy),
-                                  (float)((double)box[2 * e3 + e] / sz));
+            if (!on[j]) continue;
+            const float vx = (float)((double)box[e] / sx), vy = (float)((double)box[e3 + e] / sy),
+                        vz = (float)((double)box[2 * e3 + e] / sz);
+            if (plain[j]) {
+                const float h = 0.5f / rec[j].rho;
+                fl.fx[kk[j]] = f[j][0] + vx;
+                fl.fy[kk[j]] = f[j][1] + vy;
+                fl.fz[kk[j]] = f[j][2] + vz;
+                fl.stage[kk[j]] = StageRec<float>{rec[j].rho, add_rounded_product(rec[j].ux, vx, h),
+                                                  add_rounded_product(rec[j].uy, vy, h),
+                                                  add_rounded_product(rec[j].uz, vz, h)};
+            } else {
+                add_spread<float>(fl, kk[j], rec[j].rho, vx, vy, vz);
+            }
         }
     }
 }
@@ -1033,14 +1135,8 @@ template <typename T>
 __global__ __launch_bounds__(kIbmBlock) void k_ibm_output(IbmParams P, IbmFluid<T> fl) {
     const uint32_t total = *P.tile_count * (uint32_t)kTileNodes;
     for (uint32_t e = blockIdx.x * kIbmBlock + threadIdx.x; e < total; e += gridDim.x * kIbmBlock) {
-        const uint32_t tile = P.tile_list[e / kTileNodes];
-        const int l = (int)(e % kTileNodes);
-        const int tx = (int)(tile % (uint32_t)P.tiles[0]);
-        const int ty = (int)((tile / (uint32_t)P.tiles[0]) % (uint32_t)P.tiles[1]);
-        const int tz = (int)(tile / ((uint32_t)P.tiles[0] * (uint32_t)P.tiles[1]));
-        const int x = (tx << kTileShift) + (l & (kTileEdge - 1));
-        const int y = (ty << kTileShift) + ((l >> kTileShift) & (kTileEdge - 1));
-        const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
+        int x, y, z;
+        tile_node(P.tile_list[e / kTileNodes], (int)(e % kTileNodes), P.tiles[0], P.tiles[1], x, y, z);
         if (x >= P.nx || y >= P.ny || z >= P.nz) continue;
         const size_t k = ((size_t)z * P.ny + y) * P.nx + x;
         const StageRec<T> r = fl.stage[k];

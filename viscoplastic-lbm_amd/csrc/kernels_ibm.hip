@@ -53,6 +53,7 @@ static int stage_part(const IbmParams& P, const IbmFluid<T>& fl, int part, hipSt
             hipLaunchKernelGGL(k_ibm_pairs, waves, dim3(kContactBlock), 0, st, P);
             hipLaunchKernelGGL(k_ibm_contact_sums, waves, dim3(kContactBlock), 0, st, P);
             hipLaunchKernelGGL(k_ibm_velocity, blocks_for(ns), blk, 0, st, P);
+            if (P.box_edge > 0 && P.box_plain) hipLaunchKernelGGL(k_ibm_box_plan, waves, dim3(kContactBlock), 0, st, P);
         }
         break;
     case kIbmIteration:

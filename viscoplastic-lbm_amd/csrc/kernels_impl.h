@@ -654,8 +654,7 @@ __device__ __forceinline__ Force<T> const_force(const Params<T>& p) { return For
 // Tile of the IBM stage holding the node.
 template <typename T>
 __device__ __forceinline__ uint32_t tile_of(const Params<T>& p, int x, int y, int z) {
-    return (uint32_t)(x >> kTileShift) +
-           (uint32_t)p.tiles_x * ((uint32_t)(y >> kTileShift) + (uint32_t)p.tiles_y * (uint32_t)(z >> kTileShift));
+    return tile_index(x, y, z, p.tiles_x, p.tiles_y);
 }
 
 // omega_old: power-law state of the node (ignored otherwise).
@@ -1048,14 +1047,8 @@ __global__ __launch_bounds__(kBlock) void k_ibm_macr(const Params<typename Compu
     T* ffy = (T*)p.ibm_fy;
     T* ffz = (T*)p.ibm_fz;
     for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < total; e += gridDim.x * kBlock) {
-        const uint32_t tile = tile_list[e / kTileNodes];
-        const int l = (int)(e % kTileNodes);
-        const int tx = (int)(tile % (uint32_t)p.tiles_x);
-        const int ty = (int)((tile / (uint32_t)p.tiles_x) % (uint32_t)p.tiles_y);
-        const int tz = (int)(tile / ((uint32_t)p.tiles_x * (uint32_t)p.tiles_y));
-        const int x = (tx << kTileShift) + (l & (kTileEdge - 1));
-        const int y = (ty << kTileShift) + ((l >> kTileShift) & (kTileEdge - 1));
-        const int z = (tz << kTileShift) + (l >> (2 * kTileShift));
+        int x, y, z;
+        tile_node(tile_list[e / kTileNodes], (int)(e % kTileNodes), p.tiles_x, p.tiles_y, x, y, z);
         if (x >= p.nx || y >= p.ny || z >= p.nz) continue;
         const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
         const Node<T> nd = make_node(p, idx);

@@ -160,11 +160,30 @@ struct Params {
     int tiles_x, tiles_y;
 };
 
-// Tiles of the IBM stage: 4 x 4 x 4 nodes (a sphere's stencil shell is a few
-// nodes thick, so small tiles keep the stage's passes near the shell).
-constexpr int kTileShift = 2;
-constexpr int kTileEdge = 1 << kTileShift;
-constexpr int kTileNodes = kTileEdge * kTileEdge * kTileEdge;
+// Tiles of the IBM stage: 16 x 2 x 2 nodes. A sphere's stencil shell is a
+// few nodes thick, so small tiles keep the stage's passes near the shell; 16
+// nodes along x make a tile row one 64-byte segment of an FP32 population
+// plane (4 x 4 x 4 tiles read 16-byte rows and fetched 3.2 times the bytes
+// they used).
+constexpr int kTileShiftX = 4, kTileShiftY = 1, kTileShiftZ = 1;
+constexpr int kTileNodes = 1 << (kTileShiftX + kTileShiftY + kTileShiftZ);
+__host__ __device__ constexpr int tile_shift(int a) { return a == 0 ? kTileShiftX : (a == 1 ? kTileShiftY : kTileShiftZ); }
+// tiles along axis a of n nodes
+__host__ __device__ constexpr int tiles_along(int n, int a) { return (n + (1 << tile_shift(a)) - 1) >> tile_shift(a); }
+// tile holding node (x, y, z)
+__host__ __device__ inline uint32_t tile_index(int x, int y, int z, int tiles_x, int tiles_y) {
+    return (uint32_t)(x >> kTileShiftX) +
+           (uint32_t)tiles_x * ((uint32_t)(y >> kTileShiftY) + (uint32_t)tiles_y * (uint32_t)(z >> kTileShiftZ));
+}
+// node l (x fastest) of tile t
+__host__ __device__ inline void tile_node(uint32_t t, int l, int tiles_x, int tiles_y, int& x, int& y, int& z) {
+    const int tx = (int)(t % (uint32_t)tiles_x);
+    const int ty = (int)((t / (uint32_t)tiles_x) % (uint32_t)tiles_y);
+    const int tz = (int)(t / ((uint32_t)tiles_x * (uint32_t)tiles_y));
+    x = (tx << kTileShiftX) + (l & ((1 << kTileShiftX) - 1));
+    y = (ty << kTileShiftY) + ((l >> kTileShiftX) & ((1 << kTileShiftY) - 1));
+    z = (tz << kTileShiftZ) + (l >> (kTileShiftX + kTileShiftY));
+}
 
 // kAuxMomentsOutsideTiles: moments with the constant force for nodes outside
 // the flagged tiles; flagged tiles keep the macroscopics of the IBM stage.
