@@ -418,6 +418,68 @@ __device__ void wall_contacts(const IbmParams& P, IbmSphere& c, double f[3], dou
     }
 }
 
+// Origin of a sphere's spreading box on axis a (the box is box_edge nodes
+// wide, centred on the sphere).
+__device__ __forceinline__ int box_origin(const IbmParams& P, const IbmSphere& c, int a) {
+    return (int)floor(c.pos[a]) - (P.box_edge / 2 - 3) - 2;
+}
+
+// Whether lattice coordinate q (unwrapped, axis a) lies in the box of origin o.
+__device__ __forceinline__ bool in_box_axis(const IbmParams& P, int a, int q, int o) {
+    int d = q - o;
+    if (P.periodic[a]) d = wrap_axis(d, axis_n(P, a));
+    return d >= 0 && d < P.box_edge;
+}
+
+// Spreading plan of sphere s (by one wave; positions are fixed until the
+// stage moves the spheres): the spheres whose boxes overlap this one's
+// (at most kBoxNb listed, else the count is -1), and box_stray set when a
+// node lies farther than box_reach from its centre on some axis. Nodes within
+// box_reach put every stencil point in their own sphere's box, so with no
+// stray node a box node outside every other box receives values from its own
+// sphere's workgroup only.
+__device__ void box_plan(const IbmParams& P, int s, int lane) {
+    const IbmSphere& c = P.sph[s];
+    bool out = false;
+    for (int k = lane; k < c.n_nodes; k += 64) {
+        const int i = c.first_node + k;
+        const double p[3] = {P.px[i], P.py[i], P.pz[i]};
+        for (int a = 0; a < 3; a++) {
+            double d = p[a] - c.pos[a];
+            if (P.periodic[a]) {
+                const double n = (double)axis_n(P, a);
+                if (d > n / 2) d -= n;
+                if (d < -n / 2) d += n;
+            }
+            if (!(fabs(d) <= P.box_reach)) out = true;
+        }
+    }
+    if (__ballot(out) && lane == 0) atomicOr(P.box_stray, 1);
+    const int o[3] = {box_origin(P, c, 0), box_origin(P, c, 1), box_origin(P, c, 2)};
+    int* list = P.box_nb + s * (kBoxNb + 1);
+    int cnt = 0;
+    for (int j0 = 0; j0 < P.n_spheres; j0 += 64) {
+        const int j = j0 + lane;
+        bool ov = false;
+        if (j < P.n_spheres && j != s) {
+            const IbmSphere& cj = P.sph[j];
+            ov = true;
+            for (int a = 0; a < 3; a++) {
+                const int oj = box_origin(P, cj, a);
+                // two boxes of one width intersect when either origin lies in the other box
+                if (!in_box_axis(P, a, oj, o[a]) && !in_box_axis(P, a, o[a], oj)) ov = false;
+            }
+        }
+        const uint64_t m = __ballot(ov);
+        const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
+        if (ov && at < kBoxNb) list[1 + at] = j;
+        cnt += __popcll(m);
+    }
+    if (lane == 0) list[0] = cnt > kBoxNb ? -1 : cnt;
+}
+
+__device__ void sphere_velocity(const IbmParams& P, int s);
+
 // The contact kernels run one wave per sphere. Each lane of the wave takes
 // one cell of the sphere's neighbour range (at most 27) and walks that cell's
 // run of the sorted keys; what the lanes find is handed to lane 0 by
@@ -492,7 +554,8 @@ __global__ __launch_bounds__(1024) void k_ibm_cell_sort(IbmParams P) {
 
 // Sphere-sphere contacts of sphere i with the higher-index spheres of the
 // neighbouring cells, in ascending partner order (the reference's pair order
-// for that sphere's history); one wave per sphere.
+// for that sphere's history), and the sphere's spreading plan; one wave per
+// sphere.
 __global__ __launch_bounds__(kContactBlock) void k_ibm_pairs(IbmParams P) {
     __shared__ uint64_t sk[kMaxSortKeys];
     __shared__ int cand[kContactWaves][kMaxContacts];
@@ -500,6 +563,7 @@ __global__ __launch_bounds__(kContactBlock) void k_ibm_pairs(IbmParams P) {
     const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     const int i = (int)blockIdx.x * kContactWaves + wave;
     if (i >= P.n_spheres) return;
+    if (P.box_edge > 0 && P.box_plain) box_plan(P, i, lane);
     IbmSphere& ci = P.sph[i];
     int k, kend;
     lane_cell_run(P, sk, ci.cell, lane, k, kend);
@@ -532,7 +596,8 @@ __global__ __launch_bounds__(kContactBlock) void k_ibm_pairs(IbmParams P) {
 
 // Force / torque of the contacts in the reference's accumulation order for
 // sphere p: as the higher index (ascending partner), as the lower index
-// (ascending partner), then the walls; one wave per sphere.
+// (ascending partner), then the walls; then the sphere's velocity update.
+// One wave per sphere.
 __global__ __launch_bounds__(kContactBlock) void k_ibm_contact_sums(IbmParams P) {
     __shared__ uint64_t sk[kMaxSortKeys];
     __shared__ int from_i[kContactWaves][kMaxContacts], from_k[kContactWaves][kMaxContacts];
@@ -576,6 +641,7 @@ __global__ __launch_bounds__(kContactBlock) void k_ibm_contact_sums(IbmParams P)
     }
     if (cp.movable) wall_contacts(P, cp, f, M);
     for (int a = 0; a < 3; a++) { cp.f[a] = f[a]; cp.M[a] = M[a]; }
+    sphere_velocity(P, p);
 }
 
 // gpuUpdateParticleCenterVelocityAndRotation (ibm.cu:755-897)
@@ -759,13 +825,21 @@ __global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFlu
     const int zk = r >> 2, yj = r & 3;
     const int items = P.chunk_count[chunk] * kRowLanes;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    // position of the node of item k (the chunk's first node past the end);
-    // the next item's is loaded while the current one is interpolated
-    auto fetch = [&](int k, double (&q)[3]) {
+    // the sphere's centre and motion in registers (the node stores below
+    // could alias them in memory, so they would be reloaded every round)
+    const double cpos[3] = {c.pos[0], c.pos[1], c.pos[2]};
+    const bool movable = c.movable;
+    const double cvel[3] = {c.vel[0], c.vel[1], c.vel[2]}, cw[3] = {c.w[0], c.w[1], c.w[2]};
+    // position, area and accumulated force of the node of item k (the
+    // chunk's first node past the end); the next item's are loaded while the
+    // current one is interpolated
+    auto fetch = [&](int k, double (&q)[7]) {
         const int i = first + (k < items ? k / kRowLanes : 0);
         q[0] = P.px[i]; q[1] = P.py[i]; q[2] = P.pz[i];
+        q[3] = P.pS[i];
+        q[4] = P.pfx[i]; q[5] = P.pfy[i]; q[6] = P.pfz[i];
     };
-    double nxt[3] = {0, 0, 0};
+    double nxt[7] = {0, 0, 0, 0, 0, 0, 0};
     if (items > 0) fetch((int)threadIdx.x, nxt);
     for (int b0 = 0; b0 < items; b0 += kInterpBlock) {      // uniform trip count for the shuffles
         const int k = b0 + (int)threadIdx.x;
@@ -773,6 +847,7 @@ __global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFlu
         const int i = first + k / kRowLanes;
         double pv[4] = {0, 0, 0, 0};
         const double pos[3] = {nxt[0], nxt[1], nxt[2]};
+        const double area = nxt[3], pf[3] = {nxt[4], nxt[5], nxt[6]};
         if (b0 + kInterpBlock < items) fetch(k + kInterpBlock, nxt);
         Stencil st;
         st.valid = false;
@@ -812,22 +887,22 @@ __global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFlu
             P.pmx[i] = 0; P.pmy[i] = 0; P.pmz[i] = 0;
             continue;
         }
-        double d[3] = {pos[0] - c.pos[0], pos[1] - c.pos[1], pos[2] - c.pos[2]};
+        double d[3] = {pos[0] - cpos[0], pos[1] - cpos[1], pos[2] - cpos[2]};
         for (int a = 0; a < 3; a++) {
             if (!P.periodic[a]) continue;
             const double n = axis_n(P, a);
-            if (fabs(d[a]) > n / 2.0) d[a] = (d[a] < 0) ? (pos[a] + n) - c.pos[a] : (pos[a] - n) - c.pos[a];
+            if (fabs(d[a]) > n / 2.0) d[a] = (d[a] < 0) ? (pos[a] + n) - cpos[a] : (pos[a] - n) - cpos[a];
         }
         double uc[3] = {0, 0, 0};
-        if (c.movable) {
-            uc[0] = c.vel[0] + (c.w[1] * d[2] - c.w[2] * d[1]);
-            uc[1] = c.vel[1] + (c.w[2] * d[0] - c.w[0] * d[2]);
-            uc[2] = c.vel[2] + (c.w[0] * d[1] - c.w[1] * d[0]);
+        if (movable) {
+            uc[0] = cvel[0] + (cw[1] * d[2] - cw[2] * d[1]);
+            uc[1] = cvel[1] + (cw[2] * d[0] - cw[0] * d[2]);
+            uc[2] = cvel[2] + (cw[0] * d[1] - cw[1] * d[0]);
         }
-        const double aux = 2 * pv[0] * P.pS[i] * P.thickness;
+        const double aux = 2 * pv[0] * area * P.thickness;
         const double dF[3] = {aux * (pv[1] - uc[0]), aux * (pv[2] - uc[1]), aux * (pv[3] - uc[2])};
         const double m3[3] = {d[1] * dF[2] - d[2] * dF[1], d[2] * dF[0] - d[0] * dF[2], d[0] * dF[1] - d[1] * dF[0]};
-        P.pfx[i] += dF[0]; P.pfy[i] += dF[1]; P.pfz[i] += dF[2];
+        P.pfx[i] = pf[0] + dF[0]; P.pfy[i] = pf[1] + dF[1]; P.pfz[i] = pf[2] + dF[2];
         P.pdx[i] = dF[0]; P.pdy[i] = dF[1]; P.pdz[i] = dF[2];
         P.pmx[i] = m3[0]; P.pmy[i] = m3[1]; P.pmz[i] = m3[2];
         acc[0] += dF[0]; acc[1] += dF[1]; acc[2] += dF[2];
@@ -847,7 +922,7 @@ __global__ __launch_bounds__(kInterpBlock) void k_ibm_interp(IbmParams P, IbmFlu
     }
 }
 
-// chunk sums -> sphere force and torque, chunks in order
+// chunk sums -> sphere force and torque, chunks in order; velocity update
 __global__ __launch_bounds__(kIbmBlock) void k_ibm_add_chunks(IbmParams P) {
     const int s = blockIdx.x * kIbmBlock + threadIdx.x;
     if (s >= P.n_spheres) return;
@@ -856,69 +931,7 @@ __global__ __launch_bounds__(kIbmBlock) void k_ibm_add_chunks(IbmParams P) {
     for (int k = P.chunk0[s]; k < P.chunk0[s + 1]; k++)
         for (int a = 0; a < 6; a++) t[a] += P.chunk_sums[6 * k + a];
     for (int a = 0; a < 3; a++) { c.f[a] += t[a]; c.M[a] += t[3 + a]; }
-}
-
-// Origin of a sphere's spreading box on axis a (the box is box_edge nodes
-// wide, centred on the sphere).
-__device__ __forceinline__ int box_origin(const IbmParams& P, const IbmSphere& c, int a) {
-    return (int)floor(c.pos[a]) - (P.box_edge / 2 - 3) - 2;
-}
-
-// Whether lattice coordinate q (unwrapped, axis a) lies in the box of origin o.
-__device__ __forceinline__ bool in_box_axis(const IbmParams& P, int a, int q, int o) {
-    int d = q - o;
-    if (P.periodic[a]) d = wrap_axis(d, axis_n(P, a));
-    return d >= 0 && d < P.box_edge;
-}
-
-// Spreading plan of the stage (one wave per sphere; positions are fixed until
-// the stage moves the spheres): the spheres whose boxes overlap this one's
-// (at most kBoxNb listed, else the count is -1), and box_stray set when a
-// node lies farther than box_reach from its centre on some axis. Nodes within
-// box_reach put every stencil point in their own sphere's box, so with no
-// stray node a box node outside every other box receives values from its own
-// sphere's workgroup only.
-__global__ __launch_bounds__(kContactBlock) void k_ibm_box_plan(IbmParams P) {
-    const int lane = (int)threadIdx.x & 63;
-    const int s = (int)blockIdx.x * kContactWaves + ((int)threadIdx.x >> 6);
-    if (s >= P.n_spheres) return;
-    const IbmSphere& c = P.sph[s];
-    bool out = false;
-    for (int k = lane; k < c.n_nodes; k += 64) {
-        const int i = c.first_node + k;
-        const double p[3] = {P.px[i], P.py[i], P.pz[i]};
-        for (int a = 0; a < 3; a++) {
-            double d = p[a] - c.pos[a];
-            if (P.periodic[a]) {
-                const double n = (double)axis_n(P, a);
-                if (d > n / 2) d -= n;
-                if (d < -n / 2) d += n;
-            }
-            if (!(fabs(d) <= P.box_reach)) out = true;
-        }
-    }
-    if (__ballot(out) && lane == 0) atomicOr(P.box_stray, 1);
-    const int o[3] = {box_origin(P, c, 0), box_origin(P, c, 1), box_origin(P, c, 2)};
-    int* list = P.box_nb + s * (kBoxNb + 1);
-    int cnt = 0;
-    for (int j0 = 0; j0 < P.n_spheres; j0 += 64) {
-        const int j = j0 + lane;
-        bool ov = false;
-        if (j < P.n_spheres && j != s) {
-            const IbmSphere& cj = P.sph[j];
-            ov = true;
-            for (int a = 0; a < 3; a++) {
-                const int oj = box_origin(P, cj, a);
-                // two boxes of one width intersect when either origin lies in the other box
-                if (!in_box_axis(P, a, oj, o[a]) && !in_box_axis(P, a, o[a], oj)) ov = false;
-            }
-        }
-        const uint64_t m = __ballot(ov);
-        const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
-        if (ov && at < kBoxNb) list[1 + at] = j;
-        cnt += __popcll(m);
-    }
-    if (lane == 0) list[0] = cnt > kBoxNb ? -1 : cnt;
+    sphere_velocity(P, s);
 }
 
 // Adds the spread force v to node kk of density rho: to the force field and,

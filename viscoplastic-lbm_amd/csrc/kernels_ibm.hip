@@ -52,16 +52,15 @@ static int stage_part(const IbmParams& P, const IbmFluid<T>& fl, int part, hipSt
             hipLaunchKernelGGL(k_ibm_cell_sort, dim3(1), dim3(1024), 0, st, P);
             hipLaunchKernelGGL(k_ibm_pairs, waves, dim3(kContactBlock), 0, st, P);
             hipLaunchKernelGGL(k_ibm_contact_sums, waves, dim3(kContactBlock), 0, st, P);
-            hipLaunchKernelGGL(k_ibm_velocity, blocks_for(ns), blk, 0, st, P);
-            if (P.box_edge > 0 && P.box_plain) hipLaunchKernelGGL(k_ibm_box_plan, waves, dim3(kContactBlock), 0, st, P);
         }
         break;
     case kIbmIteration:
         if (nn) {
             hipLaunchKernelGGL((k_ibm_interp<T>), dim3(xcd_chunk_grid(P.n_chunks)), dim3(kInterpBlock), 0, st, P, fl);
-            hipLaunchKernelGGL(k_ibm_add_chunks, blocks_for(ns), blk, 0, st, P);
+            hipLaunchKernelGGL(k_ibm_add_chunks, blocks_for(ns), blk, 0, st, P);   // with the velocity update
+        } else if (ns) {
+            hipLaunchKernelGGL(k_ibm_velocity, blocks_for(ns), blk, 0, st, P);
         }
-        if (ns) hipLaunchKernelGGL(k_ibm_velocity, blocks_for(ns), blk, 0, st, P);
         if (nn && spread<T>(P, fl, st)) return -1;
         break;
     case kIbmOutput: {
