@@ -9,12 +9,14 @@
 //   contacts  uniform cell list (one-workgroup bitonic sort of (cell, sphere)
 //             in LDS), Hertz-Mindlin sphere-sphere contacts by the lower-index
 //             sphere and force / torque sums per sphere in a fixed order, one
-//             wave per sphere; walls; velocity update
-//   iteration (x max_iterations) one workgroup per sphere: interpolation and
-//             force of each node, FP64 sums of the node forces / moments
-//             (fixed-order tree); velocity update; spreading of the node
-//             forces straight into u (F / 2 rho) and the force field, through
-//             an LDS box around each sphere for FP32 fluids
+//             wave per sphere; walls; velocity update; the spreading plan
+//             (overlapping boxes, stray nodes)
+//   iteration (x max_iterations) chunks of 64 nodes of one sphere:
+//             interpolation and force of each node, FP64 sums of the node
+//             forces / moments (fixed-order tree); sphere sums and velocity
+//             update; spreading of the node forces straight into u (F / 2 rho)
+//             and the force field, through an LDS box around each sphere for
+//             FP32 fluids (one workgroup per sphere)
 //   move      sphere and node movement
 // Everything outside the flagged tiles is left alone: the reference's full-
 // lattice passes (gpuUpdateMacrIBM, gpuEulerSumIBMAuxsReset over the bordered
@@ -1087,7 +1089,7 @@ __global__ __launch_bounds__(kSphereBlock) void k_ibm_spread_box(IbmParams P, Ib
     __syncthreads();
     // flush in rounds of kFlushBatch box nodes per thread. A node that no
     // other sphere's box covers (and, with use_box, no other workgroup
-    // writes: see k_ibm_box_plan) takes plain loads and stores, the others
+    // writes: see box_plan) takes plain loads and stores, the others
     // global atomics; both add the same float values, so a node's result is
     // the same either way. The loads of a round are issued before its
     // atomics and stores, so each waits for the previous round's only
