@@ -181,6 +181,22 @@ def test_ibm_fp32_fluid_fp64_particles():
     compare_fluid(lat, o, 1e-4)
 
 
+def test_ibm_fp64_sphere_without_nodes():
+    """A sphere without surface nodes has no interpolation chunk: its velocity
+    update still runs every iteration (gravity, and a wall contact with a
+    meshed sphere), equal to the oracle's."""
+    print("case: fp64 sphere without nodes beside a meshed one, 8 steps", flush=True)
+    lat, o = make_pair(32, 24, 32, "parallelPlatesBounceBack_Y", dict(gy=-2e-5))
+    add_both(lat, o, [10.0, 9.0, 12.0], 4.0, vel=(0.001, 0, 0))
+    add_both(lat, o, [20.0, 3.8, 16.0], 4.0, vel=(0, -0.002, 0), w=(0, 0, 0.001), xyz=np.zeros((0, 3)), S=np.zeros(0))
+    lat.step(8, save_last=True)
+    o.ibm_run(8, 0)
+    compare_particles(lat, o, 1e-10, fields=("pos", "vel", "w", "f", "M"))
+    g = lat.ibm_particles()
+    assert g[1, F["vel"]][1] != -0.002            # the node-less sphere's velocity was updated
+    lat.close()
+
+
 def test_ibm_fp32_plain_flush_equals_atomic_flush():
     """The spreading flush writes box nodes that no other sphere's box covers
     with plain stores, the others with atomics. With boxes that do not overlap,
