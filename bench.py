@@ -6,9 +6,13 @@ bounce-back walls on the x and y faces, tau=0.8, yield stress 1e-5, initial
 rho=1, u=(0,0,0.01) + U(-1e-4, 1e-4) per component (seeded). One step = one
 fused gather/BC/collide/store pass over the lattice.
 
-Prints one JSON line: MLUPS over the timed steps (wall clock, inputs resident
-in HBM), the roofline of the step kernel (algorithmic 2*Q*4 = 152 B/node over
-the HIP-event time of the same steps), a CPU baseline (the oracle's OpenMP
+The lattice takes the library's automatic state layout (for this workload
+the moment coefficients: 10 per node instead of 19 populations); the other
+layout is timed beside it. Prints one JSON line: MLUPS over the timed steps
+(wall clock, inputs resident in HBM), the roofline of the step kernel that ran
+(algorithmic bytes per node update of its layout: 2*10*4 = 80 B for the
+coefficients, 2*19*4 = 152 B for populations, over the HIP-event time of the
+same steps), the same figures for the other layout, a CPU baseline (the oracle's OpenMP
 port of the reference per-node loop on the host cores, bounded sample), and
 the serving sweep: the first N jobs of the seeded Poisson stream of D3Q19
 FP32 Bingham channels (20 jobs/s; Nx U[32,256], Ny U[32,128], Nz U[16,128],
@@ -40,6 +44,10 @@ HBM_PEAK_GBS = 8000.0
 N_SIDE = 256
 Q = 19
 STORAGE_BYTES = 4
+# algorithmic bytes per node update (one read and one write of the state) and
+# the HBM traffic summary of the layout's step kernel
+LAYOUT_BYTES = {"populations": 2 * Q * STORAGE_BYTES, "moments": 2 * 10 * STORAGE_BYTES}
+TRAFFIC_FILES = {"populations": "step_kernel_traffic.json", "moments": "step_tile_kernel_traffic.json"}
 
 
 def workload_config(precision="single"):
@@ -168,55 +176,71 @@ def main():
 
     cfg = workload_config("single")
     init = initial_fields()
-    lat = vlbm.Lattice(cfg.to_desc())
-    lat.init(*init)
-    # Device settle: after the host-side set-up leaves the GPU idle, its first
-    # few dozen step launches run 10-20 % slower than the steady state (power
-    # management: 0.455 ms, rising to 0.52 ms by the tenth launch, back to
-    # 0.45 ms by the 35th), so a short warm-up would time that transient. The
-    # same step kernel runs on a second lattice of the same shape for
-    # --settle seconds first; the measured lattice still gets exactly W
-    # warm-up and K timed steps from its initial state.
-    settled_steps = 0
-    if args.settle > 0:
-        other = vlbm.Lattice(cfg.to_desc())
-        other.init()
-        t_settle = time.perf_counter()
-        while time.perf_counter() - t_settle < args.settle:
-            other.step(20)
-            other.sync()
-            settled_steps += 20
-    lat.step(args.warmup)
-    lat.sync()
-
-    t0 = time.perf_counter()
-    event_ms = lat.time_steps(args.steps)      # HIP events on the library stream around the K steps
-    lat.sync()
-    wall = time.perf_counter() - t0
-
     nodes = N_SIDE ** 3
+
+    def timed(layout_cfg, settle_s):
+        """W warm-up and K timed steps of a fresh lattice (after settle_s of
+        the same kernel on a second lattice); the wall and HIP-event times."""
+        lat = vlbm.Lattice(layout_cfg.to_desc())
+        lat.init(*init)
+        # Device settle: after the host-side set-up leaves the GPU idle, its
+        # first few dozen step launches run 10-20 % slower than the steady
+        # state (power management: 0.455 ms, rising to 0.52 ms by the tenth
+        # launch, back to 0.45 ms by the 35th), so a short warm-up would time
+        # that transient. The same step kernel runs on a second lattice of the
+        # same shape for settle_s seconds first; the measured lattice still
+        # gets exactly W warm-up and K timed steps from its initial state.
+        # The second lattice is freed only after the timed steps: freeing it
+        # idles the device for milliseconds, which brings the transient back.
+        settled = 0
+        other = None
+        if settle_s > 0:
+            other = vlbm.Lattice(layout_cfg.to_desc())
+            other.init()
+            t_settle = time.perf_counter()
+            while time.perf_counter() - t_settle < settle_s:
+                other.step(20)
+                other.sync()
+                settled += 20
+        lat.step(args.warmup)
+        lat.sync()
+        t0 = time.perf_counter()
+        event_ms = lat.time_steps(args.steps)      # HIP events on the library stream around the K steps
+        lat.sync()
+        wall = time.perf_counter() - t0
+        if other is not None:
+            other.close()
+        layout = lat.layout
+        # sanity of the simulated state after the timed steps
+        lat.step(1, save_last=True)
+        _, mean_rho = lat.reduce()
+        lat.close()
+        if not np.isfinite(mean_rho):
+            print(f"non-finite state after the timed steps ({layout})", file=sys.stderr)
+            sys.exit(1)
+        kernel_ms = event_ms / args.steps
+        achieved = LAYOUT_BYTES[layout] * nodes / (kernel_ms * 1e-3) / 1e9
+        return {"layout": layout, "wall": wall, "kernel_ms": kernel_ms, "achieved": achieved,
+                "mean_rho": mean_rho, "settled": settled}
+
+    main_run = timed(cfg, args.settle)
+    layout = main_run["layout"]
+    other_layout = "populations" if layout == "moments" else "moments"
+    other_cfg = workload_config("single")
+    other_cfg.layout = other_layout
+    other_run = timed(other_cfg, args.settle)
+
+    wall, kernel_ms, achieved = main_run["wall"], main_run["kernel_ms"], main_run["achieved"]
     mlups = nodes * args.steps / wall / 1e6
     ms_per_step = wall * 1e3 / args.steps
-    kernel_ms = event_ms / args.steps
-    bytes_per_launch = 2 * Q * STORAGE_BYTES * nodes
-    achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    settled_steps, mean_rho = main_run["settled"], main_run["mean_rho"]
     traffic = None
-    tfile = ROOT / "profiles" / "step_kernel_traffic.json"
+    tfile = ROOT / "profiles" / TRAFFIC_FILES[layout]
     if tfile.exists():
         try:
             traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
         except Exception:
             traffic = None
-
-    if args.settle > 0:
-        other.close()
-    # sanity of the simulated state after the timed steps
-    lat.step(1, save_last=True)
-    res, mean_rho = lat.reduce()
-    lat.close()
-    if not np.isfinite(mean_rho):
-        print("non-finite state after the timed steps", file=sys.stderr)
-        sys.exit(1)
 
     progress(f"{args.steps} timed steps: {kernel_ms:.5f} ms per step launch")
     cpu = None
@@ -242,7 +266,10 @@ def main():
             "workload": "D3Q19 regularized Bingham, 256x256x256, FP32, Zou-He inlet/outlet z, bounce-back x/y walls",
             "lattice": [N_SIDE, N_SIDE, N_SIDE],
             "tau": 0.8, "yield_stress": 1e-5, "u_inlet": 0.01,
-            "storage": "fp32 deviations f_i - w_i, two lattices, pull streaming",
+            "layout": layout,
+            "storage": ("fp32 regularized-state coefficients (10 per node), two lattices, pull streaming through "
+                        "an LDS plane ring" if layout == "moments" else
+                        "fp32 deviations f_i - w_i, two lattices, pull streaming"),
             "device_settle": f"{settled_steps} step launches ({args.settle:g} s) on a separate 256^3 lattice "
                              "before the warm-up",
         },
@@ -254,7 +281,14 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "kernel_ms": round(kernel_ms, 5),
-            "algorithmic_bytes_per_node": 2 * Q * STORAGE_BYTES,
+            "algorithmic_bytes_per_node": LAYOUT_BYTES[layout],
+        },
+        "other_layout": {
+            "layout": other_layout,
+            "mlups": round(nodes * args.steps / other_run["wall"] / 1e6, 1),
+            "kernel_ms": round(other_run["kernel_ms"], 5),
+            "algorithmic_bytes_per_node": LAYOUT_BYTES[other_layout],
+            "roofline_frac": round(other_run["achieved"] / HBM_PEAK_GBS, 4),
         },
         "cpu_baseline": cpu,
         "sweep": sweep_out,

@@ -242,6 +242,11 @@ int vlbm_sync(vlbm_h h);
 /* Device bytes held by the handle. */
 int vlbm_device_bytes(vlbm_h h, size_t* bytes);
 
+/* The state representation in use, VLBM_LAYOUT_POPULATIONS or
+ * VLBM_LAYOUT_MOMENTS (what VLBM_LAYOUT_AUTO chose). No counterpart in the
+ * reference, whose state is always populations. */
+int vlbm_layout(vlbm_h h, int* layout);
+
 /* Average time of a plain device-to-device copy of `bytes` (16-byte loads and
  * stores), the calibration of the HBM traffic counters. No reference
  * counterpart (measurement helper). */

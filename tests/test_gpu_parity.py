@@ -84,7 +84,7 @@ CASES_FP64 = [
     # before the row pull, for a node at both ends and for two adjacent ends
     ("closed_box_nx1_rows", dict(NX=1, NY=7, NZ=6, tau=0.7, builder="closedBoxBounceBack"), 60, 1e-2),
     ("closed_box_nx2_rows", dict(NX=2, NY=7, NZ=6, tau=0.7, builder="closedBoxBounceBack"), 60, 1e-2),
-    ("square_duct_zouhe_c2",dict(NX=12, NY=10, NZ=16, tau=0.8, builder="squareDuctZouHe",
+    ("square_duct_zouhe_c2", dict(NX=12, NY=10, NZ=16, tau=0.8, builder="squareDuctZouHe",
                                   non_newtonian_model="BINGHAM", yield_stress=1e-5,
                                   uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0]), 150, 1e-4),
     # the benchmarked kernel: NX = 16 divides the 256-node block, so the step pulls through the LDS row
@@ -429,6 +429,31 @@ def test_fp32_density_deviation_within_the_reference_single_precision():
     print(f"rho-1 vs FP64 oracle after 600 steps: FP32 GPU {e_gpu:.2e}, reference FP32 build {e_ref32:.2e}", flush=True)
     assert e_gpu <= e_ref32 and e_gpu <= 1e-4
     lat.close()
+
+
+def test_automatic_layout_choice():
+    """VLBM_LAYOUT_AUTO picks the representation measured faster for the case:
+    coefficients for float D3Q19 box maps and D3Q27 with NX >= 64,
+    populations for node-mask maps, FP64 and small lattices; vlbm_layout
+    reports the choice and explicit layouts are kept."""
+    announce("automatic layout choice")
+    def layout(**kw):
+        lat = vlbm.Lattice(base_cfg(**kw).to_desc())
+        out = lat.layout
+        lat.close()
+        return out
+    box = dict(NX=64, NY=16, NZ=8, tau=0.8, builder="squareDuctZouHe", uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0])
+    assert layout(precision="single", non_newtonian_model="BINGHAM", yield_stress=1e-5, **box) == "moments"
+    assert layout(precision="single", **box) == "moments"
+    assert layout(precision="single", non_newtonian_model="POWERLAW", n_index=0.6, k_consistency=0.05,
+                  tau_clamp=[0.51, 2.0], **box) == "moments"
+    assert layout(precision="double", **box) == "populations"
+    assert layout(precision="single", **dict(box, NX=32)) == "populations"
+    assert layout(precision="single", layout="populations", **box) == "populations"
+    assert layout(precision="single", NX=64, NY=64, NZ=4, tau=0.8, FZ=1e-5,
+                  builder="circularDuctInterpBounceBack") == "populations"
+    assert layout(precision="single", vel_set="D3Q27", NX=64, NY=8, NZ=8, tau=0.8,
+                  builder="parallelPlatesBounceBack_Y") == "moments"
 
 
 def test_c2_layouts_agree_full_size():

@@ -357,11 +357,13 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
         release(h);
         return fail(VLBM_ERR_DEVICE, "hipEventCreate failed");
     }
-    // AUTO: coefficients where they were measured faster on MI355X (D3Q27 in
-    // float storage: 28.5 vs 25.0 GLUPS at 256^3); D3Q19 runs level and FP64
-    // is faster with populations
-    const bool mom_auto = d->collision == VLBM_COLLISION_REGULARIZED && d->precision == VLBM_FP32 && d->q == 27 &&
-                          d->nx >= 64;
+    // AUTO: coefficients where they were measured faster on MI355X at 256^3
+    // in float storage: D3Q27 (power-law plates 29.7 vs 24.8 GLUPS) and
+    // D3Q19 with the box map (Bingham duct 42.1 vs 38.2, Bingham plates 44.4
+    // vs 39.4, power-law plates 37.7 vs 37.1); D3Q19 node-mask maps and FP64
+    // run faster with populations
+    const bool mom_auto = d->collision == VLBM_COLLISION_REGULARIZED && d->precision == VLBM_FP32 && d->nx >= 64 &&
+                          (d->q == 27 || !h->mask_geom);
     h->mom = d->layout == VLBM_LAYOUT_MOMENTS || (d->layout == VLBM_LAYOUT_AUTO && mom_auto);
     h->planes = h->mom ? kCoef : d->q;
     const size_t lat_bytes = (size_t)h->planes * (size_t)h->stride * h->sbytes;
@@ -969,6 +971,12 @@ int vlbm_copy_probe(size_t bytes, int iters, double* ms) {
 int vlbm_device_bytes(vlbm_h h, size_t* bytes) {
     if (!h || !bytes) return fail(VLBM_ERR_INVALID, "NULL argument");
     *bytes = h->device_bytes;
+    return VLBM_OK;
+}
+
+int vlbm_layout(vlbm_h h, int* layout) {
+    if (!h || !layout) return fail(VLBM_ERR_INVALID, "NULL argument");
+    *layout = h->mom ? VLBM_LAYOUT_MOMENTS : VLBM_LAYOUT_POPULATIONS;
     return VLBM_OK;
 }
 

@@ -436,6 +436,13 @@ struct GlobalPC {
         if (IN == kRepPop) return load_pop<S, T>(p, i, node);
         return recon<Q, T, S>(i, (const S*)p.src + node, p.stride);
     }
+    // coefficient states: the node's own 10 coefficients, read once for the
+    // several populations a bounce-back rebuilds from them
+    static constexpr bool kCoefs = IN == kRepMom;
+    __device__ __forceinline__ void own(T* c) const {
+#pragma unroll
+        for (int k = 0; k < kCoef; k++) c[k] = (T)((const S*)p.src)[(int64_t)k * p.stride + nd.idx];
+    }
     // Population i of this node in the idle lattice (its post-collision value
     // of the step before the last); read before the step overwrites it.
     __device__ __forceinline__ T held(int i) const {
@@ -590,9 +597,17 @@ __device__ __forceinline__ void apply_bc(T* g, const Params<T>& p, const Node<T>
         if (dir >= dNW && w_geom(word)) return;
         if (dir > dSEB) return;
         const uint32_t m = (Q == 19) ? c_bb19.m[dir] : c_bb27.m[dir];
+        if constexpr (PC::kCoefs && Q == 19) {
+            T c[kCoef];
+            pc.own(c);
 #pragma unroll
-        for (int i = 1; i < Q; i++)
-            if (m & (1u << i)) g[i] = pc(opp(i), 0, 0, 0);
+            for (int i = 1; i < Q; i++)
+                if (m & (1u << i)) g[i] = recon<Q, T, T>(opp(i), c, 1);
+        } else {
+#pragma unroll
+            for (int i = 1; i < Q; i++)
+                if (m & (1u << i)) g[i] = pc(opp(i), 0, 0, 0);
+        }
     } else if (scheme == kFreeSlip) {
         // specular reflection: incoming i comes from the mirror population m
         // of the node displaced by minus the tangential part of c_i

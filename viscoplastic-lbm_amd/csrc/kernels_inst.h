@@ -19,10 +19,11 @@ struct Inst {
     static void step_mom(bool direct, int in_rep, const Params<T>& p, hipStream_t st) {
         if constexpr (kHasMom) {
             if (!direct) {
-                using SH = TileShape<T>;
+                using SH = TileShape<T, Q, MASK>;
                 const int zc = tile_zchunk(p.nx, p.ny, p.nz, SH::TY);
-                const uint32_t blocks = (uint32_t)((p.nx + kTileX - 1) / kTileX) * (uint32_t)((p.ny + SH::TY - 1) / SH::TY) *
-                                        (uint32_t)((p.nz + zc - 1) / zc);
+                const uint32_t tiles = (uint32_t)((p.nx + kTileX - 1) / kTileX) * (uint32_t)((p.ny + SH::TY - 1) / SH::TY) *
+                                       (uint32_t)((p.nz + zc - 1) / zc);
+                const uint32_t blocks = 8u * ((tiles + 7u) / 8u);     // XCD order of the tiles (see the kernel)
                 hipLaunchKernelGGL((k_step_tile<Q, T, RHEO, MASK, IBM>), dim3(blocks), dim3(SH::NT), 0, st, p, zc);
             } else if (in_rep == kRepMom) {
                 hipLaunchKernelGGL((k_step<Q, S, RHEO, MASK, true, IBM, kRepMom, kRepMom>), grid_of(p.n), dim3(kBlock),

@@ -14,9 +14,17 @@
 namespace vlbm {
 
 constexpr int kTileX = 64;
-template <typename T> struct TileShape {
-    static constexpr int TY = sizeof(T) == 4 ? 4 : 2;     // LDS ring of 47.5 KB (FP32) / 63.4 KB (FP64)
+// Tile rows: 8 for float D3Q19 box-map kernels (a 79.2 KB ring, 512 threads:
+// their 123-131 VGPRs give 4 waves per SIMD with two workgroups per CU, and
+// the y halo costs 25 % instead of 50 %), 4 for other float kernels (47.5 KB;
+// their 145-166 VGPRs at 64 x 8 would leave one 512-thread workgroup per
+// CU), 2 for double (63.4 KB).
+template <typename T, int Q, bool MASK> struct TileShape {
+    static constexpr int TY = sizeof(T) == 4 ? ((Q == 19 && !MASK) ? 8 : 4) : 2;
     static constexpr int NT = kTileX * TY;
+    // waves per SIMD the register allocation must allow (the second launch
+    // bound): 4 for 64 x 8 tiles (two workgroups per CU, at most 128 VGPRs)
+    static constexpr int MIN_WAVES = TY == 8 ? 4 : 1;
     static constexpr int HX = kTileX + 2, HY = TY + 2, HP = HX * HY;
     static constexpr int PER = (HP + NT - 1) / NT;       // halo positions per thread
 };
@@ -26,16 +34,25 @@ template <typename T> struct TileShape {
 // packed 4 bits each into one int (slot of plane z+dz at bit 4 (dz+1)): a
 // select among three struct fields becomes an indexed access to the struct
 // in private memory, a shift does not.
-template <int Q, typename T>
+template <int Q, typename T, bool MASK>
 struct TilePC {
     const T* sm;
     int slots;
     int base;
     __device__ __forceinline__ T operator()(int i, int dx, int dy, int dz) const {
-        using SH = TileShape<T>;
+        using SH = TileShape<T, Q, MASK>;
         const int slot = (slots >> (4 * (dz + 1))) & 15;
         const T* q = sm + slot * (kCoef * SH::HP) + base + dy * SH::HX + dx;
         return recon<Q, T, T>(i, q, SH::HP);
+    }
+    // the node's own 10 coefficients, read once for the several populations a
+    // bounce-back rebuilds from them
+    static constexpr bool kCoefs = true;
+    __device__ __forceinline__ void own(T* c) const {
+        using SH = TileShape<T, Q, MASK>;
+        const T* q = sm + ((slots >> 4) & 15) * (kCoef * SH::HP) + base;
+#pragma unroll
+        for (int k = 0; k < kCoef; k++) c[k] = q[k * SH::HP];
     }
     // the coefficient layout is refused for the rheometer, the one reader of held values
     __device__ __forceinline__ T held(int) const { return (T)0; }
@@ -47,14 +64,23 @@ __device__ __forceinline__ int wrap_mod(int v, int n) {
 }
 
 // One step over the lattice, coefficient planes src -> dst (p.src, p.dst).
-// Grid: ceil(nx/TX) * ceil(ny/TY) * ceil(nz/zc) workgroups of TX*TY threads.
+// Grid: 8 ceil(n / 8) workgroups of TX*TY threads for the n = ceil(nx/TX) *
+// ceil(ny/TY) * ceil(nz/zc) tiles.
 template <int Q, typename T, int RHEO, bool MASK, bool IBM>
-__global__ __launch_bounds__(TileShape<T>::NT) void k_step_tile(const Params<T> p, int zc) {
-    using SH = TileShape<T>;
+__global__ __launch_bounds__((TileShape<T, Q, MASK>::NT), (TileShape<T, Q, MASK>::MIN_WAVES)) void k_step_tile(const Params<T> p,
+                                                                                                     int zc) {
+    using SH = TileShape<T, Q, MASK>;
     constexpr int TX = kTileX, TY = SH::TY, NT = SH::NT, HX = SH::HX, HP = SH::HP, PER = SH::PER;
     __shared__ T sm[3 * kCoef * HP];
     const int ntx = (p.nx + TX - 1) / TX, nty = (p.ny + TY - 1) / TY;
-    const int b = blockIdx.x;
+    // Workgroups go to the 8 XCDs in turn (blockIdx mod 8), each XCD with its
+    // own L2. Block b takes tile (b mod 8) ceil(n / 8) + b / 8 of the n tiles,
+    // so x- and y-neighbouring tiles run on one XCD at about the same time
+    // and share the cache lines of their halos (a tile row's x halo alone
+    // fetches 2 extra lines). The grid is 8 ceil(n / 8) blocks.
+    const int n_tiles = ntx * nty * ((p.nz + zc - 1) / zc);
+    const int b = (int)(blockIdx.x % 8u) * ((n_tiles + 7) / 8) + (int)(blockIdx.x / 8u);
+    if (b >= n_tiles) return;
     const int x0 = (b % ntx) * TX, y0 = ((b / ntx) % nty) * TY, z0 = (b / (ntx * nty)) * zc;
     const int z1 = min(z0 + zc, p.nz);
     const int t = threadIdx.x, lx = t % TX, ly = t / TX;
@@ -114,7 +140,7 @@ __global__ __launch_bounds__(TileShape<T>::NT) void k_step_tile(const Params<T> 
             nd.idx = (uint32_t)z * plane + (uint32_t)y * (uint32_t)p.nx + (uint32_t)x;
             const uint32_t word = node_word<MASK>(p, nd);
             if (w_used(word)) {
-                const TilePC<Q, T> pc{sm, s_m | (s_0 << 4) | (s_p << 8), (ly + 1) * HX + (lx + 1)};
+                const TilePC<Q, T, MASK> pc{sm, s_m | (s_0 << 4) | (s_p << 8), (ly + 1) * HX + (lx + 1)};
                 T g[Q];
                 gather<Q, T>(g, pc);
                 if (w_scheme(word) != kNull) apply_bc<Q, T, MASK>(g, p, nd, word, pc);

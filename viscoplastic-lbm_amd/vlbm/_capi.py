@@ -110,6 +110,7 @@ def _load() -> C.CDLL:
         "vlbm_time_steps": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "vlbm_sync": (C.c_int, [C.c_void_p]),
         "vlbm_device_bytes": (C.c_int, [C.c_void_p, P(C.c_size_t)]),
+        "vlbm_layout": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "vlbm_copy_probe": (C.c_int, [C.c_size_t, C.c_int, dp]),
         "vlbm_ibm_desc_defaults": (None, [P(VlbmIbmDesc)]),
         "vlbm_ibm_enable": (C.c_int, [C.c_void_p, P(VlbmIbmDesc)]),

@@ -138,6 +138,13 @@ class Lattice:
         check(lib.vlbm_device_bytes(self._h, C.byref(b)))
         return b.value
 
+    @property
+    def layout(self) -> str:
+        """State representation in use: "populations" or "moments"."""
+        v = C.c_int()
+        check(lib.vlbm_layout(self._h, C.byref(v)))
+        return "moments" if v.value == _capi.LAYOUT_MOMENTS else "populations"
+
     # -- macroscopics -----------------------------------------------------------
     def compute_macroscopics(self) -> None:
         check(lib.vlbm_compute_macroscopics(self._h))
