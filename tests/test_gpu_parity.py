@@ -433,27 +433,29 @@ def test_fp32_density_deviation_within_the_reference_single_precision():
 
 def test_automatic_layout_choice():
     """VLBM_LAYOUT_AUTO picks the representation measured faster for the case:
-    coefficients for float D3Q19 box maps and D3Q27 with NX >= 64,
-    populations for node-mask maps, FP64 and small lattices; vlbm_layout
-    reports the choice and explicit layouts are kept."""
+    coefficients for float D3Q19 box maps and D3Q27 large enough for 2,048
+    tiles (NX >= 64), populations for node-mask maps, FP64 and smaller
+    lattices; vlbm_layout reports the choice and explicit layouts are kept."""
     announce("automatic layout choice")
     def layout(**kw):
         lat = vlbm.Lattice(base_cfg(**kw).to_desc())
         out = lat.layout
         lat.close()
         return out
-    box = dict(NX=64, NY=16, NZ=8, tau=0.8, builder="squareDuctZouHe", uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0])
+    # 2 x 32 x 32 = 2,048 tiles of 64 x 8 nodes and 4 planes
+    box = dict(NX=128, NY=256, NZ=128, tau=0.8, builder="squareDuctZouHe", uz_bc=[0, 0.01, 0, 0, 0, 0, 0, 0])
     assert layout(precision="single", non_newtonian_model="BINGHAM", yield_stress=1e-5, **box) == "moments"
     assert layout(precision="single", **box) == "moments"
     assert layout(precision="single", non_newtonian_model="POWERLAW", n_index=0.6, k_consistency=0.05,
                   tau_clamp=[0.51, 2.0], **box) == "moments"
     assert layout(precision="double", **box) == "populations"
     assert layout(precision="single", **dict(box, NX=32)) == "populations"
+    assert layout(precision="single", **dict(box, NZ=124)) == "populations"    # 1,984 tiles
     assert layout(precision="single", layout="populations", **box) == "populations"
-    assert layout(precision="single", NX=64, NY=64, NZ=4, tau=0.8, FZ=1e-5,
+    assert layout(precision="single", NX=128, NY=128, NZ=256, tau=0.8, FZ=1e-5,
                   builder="circularDuctInterpBounceBack") == "populations"
-    assert layout(precision="single", vel_set="D3Q27", NX=64, NY=8, NZ=8, tau=0.8,
-                  builder="parallelPlatesBounceBack_Y") == "moments"
+    assert layout(precision="single", vel_set="D3Q27", NX=128, NY=128, NZ=128, tau=0.8,
+                  builder="parallelPlatesBounceBack_Y") == "moments"          # 2 x 32 x 32 tiles of 64 x 4
 
 
 def test_c2_layouts_agree_full_size():

@@ -361,9 +361,14 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
     // in float storage: D3Q27 (power-law plates 29.7 vs 24.8 GLUPS) and
     // D3Q19 with the box map (Bingham duct 42.1 vs 38.2, Bingham plates 44.4
     // vs 39.4, power-law plates 37.7 vs 37.1); D3Q19 node-mask maps and FP64
-    // run faster with populations
+    // run faster with populations. The tile kernel needs about 2,048
+    // workgroups (64 x 8 or 64 x 4 tiles, z-chunks of at least 4 planes) to
+    // fill the device; smaller lattices keep populations (one thread per
+    // node).
+    const int tile_rows = (d->q == 19 && !h->mask_geom) ? 8 : 4;
+    const long tiles = (long)((d->nx + 63) / 64) * ((d->ny + tile_rows - 1) / tile_rows) * ((d->nz + 3) / 4);
     const bool mom_auto = d->collision == VLBM_COLLISION_REGULARIZED && d->precision == VLBM_FP32 && d->nx >= 64 &&
-                          (d->q == 27 || !h->mask_geom);
+                          (d->q == 27 || !h->mask_geom) && tiles >= 2048;
     h->mom = d->layout == VLBM_LAYOUT_MOMENTS || (d->layout == VLBM_LAYOUT_AUTO && mom_auto);
     h->planes = h->mom ? kCoef : d->q;
     const size_t lat_bytes = (size_t)h->planes * (size_t)h->stride * h->sbytes;

@@ -43,8 +43,8 @@ def centres(n, seed=2024, shape=SHAPE, r=RADIUS):
     return np.array(out)
 
 
-def make(precision, n_spheres, seed):
-    cfg = SimConfig(precision=precision, NX=SHAPE[0], NY=SHAPE[1], NZ=SHAPE[2], tau=0.6, vel_set="D3Q19",
+def make(precision, n_spheres, seed, layout="auto"):
+    cfg = SimConfig(precision=precision, layout=layout, NX=SHAPE[0], NY=SHAPE[1], NZ=SHAPE[2], tau=0.6, vel_set="D3Q19",
                     use_ibm=True, non_newtonian_model=None, sim_id=0, path_save="bench", steps=5000,
                     macr_save=0, data_report=0, builder="closedBoxBounceBack")
     lat = Lattice(cfg.to_desc())
@@ -59,9 +59,10 @@ def make(precision, n_spheres, seed):
     return lat
 
 
-def run(precision, steps, warmup, n_spheres, seed):
+def run(precision, steps, warmup, n_spheres, seed, layout="auto"):
     print(f"[ibm_bench] {precision}: set-up of {n_spheres} spheres", file=sys.stderr, flush=True)
-    lat = make(precision, n_spheres, seed)
+    lat = make(precision, n_spheres, seed, layout)
+    used = lat.layout
     lat.step(warmup)
     lat.sync()
     print(f"[ibm_bench] {precision}: timing {steps} steps", file=sys.stderr, flush=True)
@@ -71,7 +72,7 @@ def run(precision, steps, warmup, n_spheres, seed):
     lat.ibm_status()
     rec = lat.ibm_particles()
     lat.close()
-    return rec, wall, ms_lbm, ms_ibm
+    return rec, wall, ms_lbm, ms_ibm, used
 
 
 def main():
@@ -81,9 +82,11 @@ def main():
     ap.add_argument("--spheres", type=int, default=200)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--layout", default="auto", choices=["auto", "populations", "moments"],
+                    help="state layout of the FP32 lattice (auto: the library's choice)")
     a = ap.parse_args()
     nodes = SHAPE[0] * SHAPE[1] * SHAPE[2]
-    rec32, wall, ms_lbm, ms_ibm = run("single", a.steps, a.warmup, a.spheres, a.seed)
+    rec32, wall, ms_lbm, ms_ibm, used = run("single", a.steps, a.warmup, a.spheres, a.seed, a.layout)
     total = a.warmup + a.steps
     out = {
         "metric": "MLUPS, IBM particle-laden box", "value": round(nodes * a.steps / wall / 1e6, 1),
@@ -92,10 +95,10 @@ def main():
         "ms_lbm_per_step": round(ms_lbm / a.steps, 4), "ms_ibm_per_step": round(ms_ibm / a.steps, 4),
         "dtype": "f32 fluid, f64 particles", "data": "synthetic",
         "config": {"workload": "IBM box 256x256x512 D3Q19 tau 0.6, 200 spheres r 8, 804 Fibonacci nodes, "
-                               "3 iterations", "spheres": a.spheres, "seed": a.seed},
+                               "3 iterations", "spheres": a.spheres, "seed": a.seed, "layout": used},
     }
     if not a.no_fp64:
-        rec64, wall64, _, _ = run("double", a.steps, a.warmup, a.spheres, a.seed)
+        rec64, wall64, _, _, _ = run("double", a.steps, a.warmup, a.spheres, a.seed)
         d = np.linalg.norm(rec32[:, 0:3] - rec64[:, 0:3], axis=1)
         travel = np.linalg.norm(rec64[:, 0:3] - centres(a.spheres, a.seed), axis=1)
         out["trajectory_deviation"] = {
