@@ -153,12 +153,16 @@ __global__ __launch_bounds__((TileShape<T, Q, MASK>::NT), (TileShape<T, Q, MASK>
     }
 }
 
-// z-chunk per workgroup: 32 planes, fewer when the lattice needs more
-// workgroups to fill the device.
+// z-chunk per workgroup: 32 planes, fewer until the lattice has enough
+// workgroups to fill the device in whole rounds: 2,048 for 64 x 8 tiles (two
+// per CU: 4 rounds), 4,096 for 64 x 4 and 64 x 2 tiles (at 256^3 the 64 x 4
+// D3Q27 kernel, three per CU, ran 29.7 GLUPS with 2,048 workgroups of 32
+// planes, 31.0 with 4,096 of 16, 25.9 with 1,024 of 64).
 inline int tile_zchunk(int nx, int ny, int nz, int ty) {
     const long tiles = (long)((nx + kTileX - 1) / kTileX) * ((ny + ty - 1) / ty);
+    const long want = ty == 8 ? 2048 : 4096;
     int zc = 32;
-    while (zc > 4 && tiles * ((nz + zc - 1) / zc) < 2048) zc /= 2;
+    while (zc > 4 && tiles * ((nz + zc - 1) / zc) < want) zc /= 2;
     return zc < nz ? zc : nz;
 }
 
