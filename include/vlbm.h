@@ -53,8 +53,10 @@ extern "C" {
  * MOMENTS: 10 planes of coefficients (A, B, G) with g*_i = w_i (A + B.c_i +
  * c_i^T G c_i), which hold the regularized post-collision state exactly (float
  * and double storage, regularized collision only): 10 instead of Q values
- * read and written per node. AUTO: MOMENTS for D3Q27 in float storage with
- * NX >= 64, else POPULATIONS. */
+ * read and written per node. AUTO: MOMENTS in float storage with the
+ * regularized collision for D3Q27, and for D3Q19 with a box-builder map, when
+ * NX >= 64 and the lattice holds at least 2,048 tiles of 64 x 8 nodes (64 x 4
+ * for D3Q27) by 4 planes; else POPULATIONS. vlbm_layout reports the choice. */
 #define VLBM_LAYOUT_AUTO 0
 #define VLBM_LAYOUT_POPULATIONS 1
 #define VLBM_LAYOUT_MOMENTS 2
