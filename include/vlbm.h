@@ -54,9 +54,10 @@ extern "C" {
  * c_i^T G c_i), which hold the regularized post-collision state exactly (float
  * and double storage, regularized collision only): 10 instead of Q values
  * read and written per node. AUTO: MOMENTS in float storage with the
- * regularized collision for D3Q27, and for D3Q19 with a box-builder map, when
- * NX >= 64 and the lattice holds at least 2,048 tiles of 64 x 8 nodes (64 x 4
- * for D3Q27) by 4 planes; else POPULATIONS. vlbm_layout reports the choice. */
+ * regularized collision, for every builder but the rheometer, when NX >= 64
+ * and the lattice holds at least 2,048 tiles of 64 x 8 nodes (64 x 4 for
+ * D3Q27 and for the duct builders' node maps) by 4 planes; else POPULATIONS.
+ * vlbm_layout reports the choice. */
 #define VLBM_LAYOUT_AUTO 0
 #define VLBM_LAYOUT_POPULATIONS 1
 #define VLBM_LAYOUT_MOMENTS 2

@@ -433,9 +433,10 @@ def test_fp32_density_deviation_within_the_reference_single_precision():
 
 def test_automatic_layout_choice():
     """VLBM_LAYOUT_AUTO picks the representation measured faster for the case:
-    coefficients for float D3Q19 box maps and D3Q27 large enough for 2,048
-    tiles (NX >= 64), populations for node-mask maps, FP64 and smaller
-    lattices; vlbm_layout reports the choice and explicit layouts are kept."""
+    coefficients for float lattices large enough for 2,048 tiles (NX >= 64),
+    box maps and duct node maps alike, populations for the rheometer (whose
+    plates hold populations), FP64 and smaller lattices; vlbm_layout reports
+    the choice and explicit layouts are kept."""
     announce("automatic layout choice")
     def layout(**kw):
         lat = vlbm.Lattice(base_cfg(**kw).to_desc())
@@ -452,8 +453,14 @@ def test_automatic_layout_choice():
     assert layout(precision="single", **dict(box, NX=32)) == "populations"
     assert layout(precision="single", **dict(box, NZ=124)) == "populations"    # 1,984 tiles
     assert layout(precision="single", layout="populations", **box) == "populations"
-    assert layout(precision="single", NX=128, NY=128, NZ=256, tau=0.8, FZ=1e-5,
-                  builder="circularDuctInterpBounceBack") == "populations"
+    # 2 x 32 x 64 tiles of 64 x 4 nodes with the duct's node map; 2 x 32 x 31 = 1,984 keep populations
+    duct = dict(NX=128, NY=128, NZ=256, tau=0.8, FZ=1e-5, builder="circularDuctInterpBounceBack")
+    assert layout(precision="single", **duct) == "moments"
+    assert layout(precision="single", **dict(duct, builder="annularDuctInterpBounceBack")) == "moments"
+    assert layout(precision="single", **dict(duct, NZ=124)) == "populations"
+    assert layout(precision="double", **duct) == "populations"
+    assert layout(precision="single", NX=128, NY=128, NZ=256, tau=0.8, builder="parallelPlatesRheometer",
+                  outer_rotation=0.003) == "populations"
     assert layout(precision="single", vel_set="D3Q27", NX=128, NY=128, NZ=128, tau=0.8,
                   builder="parallelPlatesBounceBack_Y") == "moments"          # 2 x 32 x 32 tiles of 64 x 4
 

@@ -357,18 +357,21 @@ int vlbm_create(const vlbm_desc* d, vlbm_h* out) {
         release(h);
         return fail(VLBM_ERR_DEVICE, "hipEventCreate failed");
     }
-    // AUTO: coefficients where they were measured faster on MI355X at 256^3
-    // in float storage: D3Q27 (power-law plates 29.7 vs 24.8 GLUPS) and
-    // D3Q19 with the box map (Bingham duct 42.1 vs 38.2, Bingham plates 44.4
-    // vs 39.4, power-law plates 37.7 vs 37.1); D3Q19 node-mask maps and FP64
-    // run faster with populations. The tile kernel needs about 2,048
-    // workgroups (64 x 8 or 64 x 4 tiles, z-chunks of at least 4 planes) to
-    // fill the device; smaller lattices keep populations (one thread per
-    // node).
+    // AUTO: coefficients in float storage, where they were measured faster on
+    // MI355X (GLUPS, coefficients vs populations, 256^3 unless stated): D3Q19
+    // box maps (Bingham duct 42.1 vs 38.2, Bingham plates 44.4 vs 39.4,
+    // power-law plates 37.7 vs 37.1), D3Q19 ducts with the node map
+    // (circular 32.6 vs 30.8, power-law circular 28.5 vs 27.9, annular 32.0
+    // vs 29.1; circular 128^3 25.5 vs 22.6, 64 x 64 x 512 22.5 vs 16.5) and
+    // D3Q27 (power-law plates 31.0 vs 24.8, circular duct 29.2 vs 21.6). FP64
+    // runs faster with populations, and the rheometer's plates hold
+    // populations. The tile kernel needs about 2,048 workgroups (64 x 8 or
+    // 64 x 4 tiles, z-chunks of at least 4 planes) to fill the device;
+    // smaller lattices keep populations (one thread per node).
     const int tile_rows = (d->q == 19 && !h->mask_geom) ? 8 : 4;
     const long tiles = (long)((d->nx + 63) / 64) * ((d->ny + tile_rows - 1) / tile_rows) * ((d->nz + 3) / 4);
     const bool mom_auto = d->collision == VLBM_COLLISION_REGULARIZED && d->precision == VLBM_FP32 && d->nx >= 64 &&
-                          (d->q == 27 || !h->mask_geom) && tiles >= 2048;
+                          d->builder != VLBM_BUILDER_RHEOMETER && tiles >= 2048;
     h->mom = d->layout == VLBM_LAYOUT_MOMENTS || (d->layout == VLBM_LAYOUT_AUTO && mom_auto);
     h->planes = h->mom ? kCoef : d->q;
     const size_t lat_bytes = (size_t)h->planes * (size_t)h->stride * h->sbytes;
