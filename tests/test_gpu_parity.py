@@ -491,6 +491,44 @@ def test_c2_layouts_agree_full_size():
         assert rel_l2(u, u0) <= tol, key
 
 
+@pytest.mark.parametrize("builder, extra", [
+    ("circularDuctInterpBounceBack", dict(non_newtonian_model="BINGHAM", yield_stress=1e-5)),
+    ("annularDuctInterpBounceBack", dict(inner_rotation=0.002)),
+])
+def test_duct_layouts_agree_at_automatic_size(builder, extra):
+    """The duct builders' node maps take the coefficient layout by default from
+    2,048 tiles on. At 128 x 128 x 256 (4,096 tiles of 64 x 4 nodes by 4
+    planes) after 300 steps the FP64 coefficient layout equals the FP64
+    population layout on the colliding nodes (f - w and u, FP64 bar 1e-10),
+    and the automatic FP32 lattice, which runs coefficients, stays within 1e-4
+    of FP64 (f - w, rho - 1, u)."""
+    announce(f"{builder} 128x128x256 layouts, 300 steps")
+    kw = dict(NX=128, NY=128, NZ=256, tau=0.8, FZ=1e-5, builder=builder, **extra)
+    shape = (256, 128, 128)
+    init = (np.ones(shape),) + seeded_velocity(shape, (0, 0, 0), 1e-4, seed=21)
+    out = {}
+    for prec, layout in (("double", "populations"), ("double", "moments"), ("single", "auto")):
+        lat = vlbm.Lattice(base_cfg(precision=prec, layout=layout, **kw).to_desc())
+        if layout == "auto":
+            assert lat.layout == "moments"
+        lat.init(*init)
+        lat.step(300, save_last=True)
+        lat.sync()
+        m = lat.macroscopics()
+        out[(prec, layout)] = (pops_dev(lat.get_pops(), 19), np.stack([m["ux"], m["uy"], m["uz"]]))
+        used = used_mask(lat)
+        lat.close()
+    gp0, u0 = out[("double", "populations")]
+    for key, tol in ((("double", "moments"), 1e-10), (("single", "auto"), 1e-4)):
+        gp, u = out[key]
+        e_pop = rel_l2(gp[:, used], gp0[:, used])
+        e_u = rel_l2(u[:, used], u0[:, used])
+        dr, dr0 = gp.sum(0)[used], gp0.sum(0)[used]
+        e_dr = float(np.linalg.norm(dr - dr0) / max(np.linalg.norm(dr0), np.linalg.norm(u0[:, used])))
+        print(key, f"f-w {e_pop:.2e}  rho-1 {e_dr:.2e}  u {e_u:.2e}", flush=True)
+        assert e_pop <= tol and e_dr <= tol and e_u <= tol, key
+
+
 # ---------------------------------------------------------------------------
 # driver loop (main.cu cadence) and output files
 # ---------------------------------------------------------------------------
