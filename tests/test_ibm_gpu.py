@@ -106,6 +106,40 @@ def test_ibm_fp64_contacts_walls_and_iterations(layout):
     assert np.array_equal(dims, d2) and np.array_equal(cells, c2)
 
 
+@pytest.mark.parametrize("layout", ["populations", "moments"])
+def test_ibm_fp64_spheres_in_circular_duct(layout):
+    """Node-map lattice: a circular duct (interpolated bounce-back wall of
+    radius 10.5) driven along z, two moving spheres clear of the wall, one
+    across the periodic z ends, 2 iterations: 10 coupled steps. The fluid is
+    compared on the colliding nodes (the map leaves the others unspecified)."""
+    import vlbm
+    lat, o = make_pair(24, 24, 24, "circularDuctInterpBounceBack",
+                       dict(max_iterations=2, periodic_x=0, periodic_y=0, periodic_z=1), FZ=1e-5, layout=layout)
+    assert lat.layout == layout
+    add_both(lat, o, [12.0, 12.0, 12.0], 3.0, vel=(0, 0, 0.001), w=(0, 0.001, 0))
+    add_both(lat, o, [12.5, 11.0, 3.0], 3.0, vel=(0.001, 0, 0))
+    n = 10
+    print(f"case: fp64 circular duct 2 spheres, 10 steps [{layout}]", flush=True)
+    lat.step(n, save_last=True)
+    o.ibm_run(n, 0)
+    lat.ibm_status()
+    compare_particles(lat, o, 1e-10)
+    used = ((vlbm.build_map(lat.desc) >> 21) & 1).astype(bool)
+    lat.sync()
+    m, mo = lat.macroscopics(), o.macroscopics()
+    e_rho = rel_l2(m["rho"][used], mo["rho"][used])
+    e_u = rel_l2(np.stack([m[k][used] for k in ("ux", "uy", "uz")]),
+                 np.stack([mo[k][used] for k in ("ux", "uy", "uz")]))
+    fg, fo = lat.ibm_force()[:, used], o.ibm_force()[:, used]
+    e_f = rel_l2(fg - fo.mean(), fo - fo.mean())
+    print(f"fluid rel-L2 on colliding nodes: rho {e_rho:.2e}  u {e_u:.2e}  force {e_f:.2e}")
+    assert e_rho <= 1e-10 and e_u <= 1e-10 and e_f <= 1e-9
+    pg, fg, dg = lat.ibm_nodes()
+    pc, fc, dc = o.ibm_nodes()
+    assert rel(pg, pc) <= 1e-12
+    assert rel(fg, fc) <= 1e-10 and rel(dg, dc) <= 1e-10
+
+
 def test_ibm_fp64_closed_box_fibonacci_stiff_contacts():
     """Closed box, walls on every axis, four spheres with Fibonacci meshes
     made by the library, E = 1e3, gravity -z, 3 iterations: 10 steps."""
