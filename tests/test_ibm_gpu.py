@@ -215,6 +215,34 @@ def test_ibm_fp32_fluid_fp64_particles():
     compare_fluid(lat, o, 1e-4)
 
 
+@pytest.mark.parametrize("layout", ["populations", "moments"])
+def test_ibm_fp32_spheres_in_circular_duct(layout):
+    """FP32 fluid on the circular duct's node map (the layout large ducts take
+    by default: coefficients), FP64 particles, against the FP64 oracle: 10
+    coupled steps; fluid u and force field compared on the colliding nodes."""
+    import vlbm
+    lat, o = make_pair(24, 24, 24, "circularDuctInterpBounceBack",
+                       dict(max_iterations=2, periodic_x=0, periodic_y=0, periodic_z=1), precision="single",
+                       FZ=1e-5, layout=layout)
+    add_both(lat, o, [12.0, 12.0, 12.0], 3.0, vel=(0, 0, 0.001), w=(0, 0.001, 0))
+    add_both(lat, o, [12.5, 11.0, 3.0], 3.0, vel=(0.001, 0, 0))
+    print(f"case: fp32 circular duct 2 spheres, 10 steps [{layout}]", flush=True)
+    lat.step(10, save_last=True)
+    o.ibm_run(10, 0)
+    g, c = lat.ibm_particles(), o.ibm_particles()
+    assert rel(g[:, F["pos"]], c[:, F["pos"]]) <= 1e-6
+    assert rel(g[:, F["vel"]], c[:, F["vel"]]) <= 1e-3
+    used = ((vlbm.build_map(lat.desc) >> 21) & 1).astype(bool)
+    lat.sync()
+    m, mo = lat.macroscopics(), o.macroscopics()
+    e_u = rel_l2(np.stack([m[k][used] for k in ("ux", "uy", "uz")]),
+                 np.stack([mo[k][used] for k in ("ux", "uy", "uz")]))
+    fg, fo = lat.ibm_force()[:, used], o.ibm_force()[:, used]
+    e_f = rel_l2(fg - fo.mean(), fo - fo.mean())
+    print(f"fp32 vs fp64 oracle on colliding nodes: u {e_u:.2e}  force {e_f:.2e}")
+    assert e_u <= 1e-4 and e_f <= 1e-3
+
+
 def test_ibm_fp64_sphere_without_nodes():
     """A sphere without surface nodes has no interpolation chunk: its velocity
     update still runs every iteration (gravity, and a wall contact with a
